@@ -1,0 +1,1248 @@
+/*
+ * tecno_oracle.c — CPU FP64 restatement of the reference TeCNO step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker; see tecno_oracle.h).
+ *
+ * Operation order follows the reference line by line where rounding depends
+ * on it; the citations name the reference file and lines each function
+ * restates.  Build with gcc -O2 -ffp-contract=off (no FMA), as the reference
+ * is built with g++ -O3 for baseline x86-64, which has no FMA.
+ */
+#include "tecno_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define SPW_EPS 1.0e-13 /* limiter.h:14 */
+
+static double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+static double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
+
+/* ------------------------------------------------------------------------ */
+/* logarithmic mean: enflo/src/misc_func.h:14-32                            */
+double tor_logavg(double a, double b) {
+  double xi = b / a;
+  double f = (xi - 1.0) / (xi + 1.0);
+  double u = f * f;
+  double F;
+  if (u < 1.0e-2) {
+    double u2 = u * u;
+    double u3 = u2 * u;
+    F = 1.0 + u / 3.0 + u2 / 5.0 + u3 / 7.0;
+  } else {
+    F = log(xi) / 2.0 / f;
+  }
+  return 0.5 * (a + b) / F;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SP-WENO: limiter.h:103-118 (fp), 153-187 (Cp), 193-252 (sp_weno_rec)     */
+static double spw_fp(double theta0, double theta1) {
+  double psi, val;
+  if (fabs(theta0 - 1.0) < SPW_EPS)
+    psi = 0.0;
+  else
+    psi = (1 - theta1) / (1 - theta0);
+  if (fabs(theta0 - 1.0) > SPW_EPS && fabs(psi + 1.0) > SPW_EPS)
+    val = 1.0 / (1.0 + psi);
+  else
+    val = 1.0;
+  return val;
+}
+
+static void spw_cp(double theta0, double theta1, double del1, double scale,
+                   double* C, double* corr) {
+  double psi;
+  if (fabs(theta0 - 1.0) < SPW_EPS || fabs(theta1 - 1.0) < SPW_EPS)
+    psi = 0.0;
+  else
+    psi = (1 - theta1) / (1 - theta0);
+  *corr = 0.0;
+  if (fabs(psi + 1.0) > SPW_EPS && fabs(theta0 - 1.0) > SPW_EPS && psi < 0) {
+    double a = spw_fp(theta0, theta1), b = spw_fp(theta1, theta0);
+    *C = (a / (pow(a, 2.0) + pow(b, 2.0))) / 8.0;
+    *corr += -0.25 * pow(dmin(fabs(del1 / scale), fabs(del1)), 3.0) /
+             (1.0 - theta0);
+  } else if (fabs(theta0 - 1.0) > SPW_EPS && fabs(psi + 1.0) < SPW_EPS) {
+    *C = 0.0;
+  } else if (fabs(theta0 - 1.0) < SPW_EPS ||
+             (psi > -SPW_EPS && fabs(theta0) < 1 + SPW_EPS)) {
+    *C = -3.0 / 8.0;
+  } else if (psi > -SPW_EPS && fabs(theta0) > 1) {
+    *C = 1.0 / 8.0;
+  }
+  /* else: "unknown region" — C keeps the caller's value (limiter.h:180-185) */
+}
+
+void tor_sp_weno(const double s[4], int corr_type, double* recl,
+                 double* recr) {
+  double del0 = s[1] - s[0];
+  double del1 = s[2] - s[1];
+  double del2 = s[3] - s[2];
+  if (fabs(del1) < SPW_EPS) {
+    *recl = s[1];
+    *recr = s[2];
+    return;
+  }
+  double tp = del0 / del1;
+  double tm = del2 / del1;
+  double scale = 0.5 * (fabs(s[1]) + fabs(s[2]));
+  double C1 = 0, C2 = 0, C1c = 0, C2c = 0;
+  spw_cp(tp, tm, del1, scale, &C1, &C1c);
+  spw_cp(tm, tp, del1, scale, &C2, &C2c);
+  if (corr_type > 0) {
+    double C1n = C1 + C1c;
+    double C2n = C2 + C2c;
+    if (corr_type == 1) {
+      C1 = C1n;
+      C2 = C2n;
+    }
+    C1 = dmin(dmax(C1, -3.0 / 8.0), 1.0 / 8.0);
+    C2 = dmin(dmax(C2, -3.0 / 8.0), 1.0 / 8.0);
+  }
+  double w0 = 3.0 / 4.0 + 2 * C1;
+  double w0t = 1.0 / 4.0 - 2 * C2;
+  *recl = w0 * 0.5 * (s[1] + s[2]) + (1 - w0) * (-0.5 * s[0] + 1.5 * s[1]);
+  *recr = (1 - w0t) * 0.5 * (s[1] + s[2]) + w0t * (-0.5 * s[3] + 1.5 * s[2]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pointwise ENO: limiter.h:8-12 (coefficients), 63-97                       */
+static const double ENO2C[2][2] = {{-0.5, 1.5}, {0.5, 0.5}};
+static const double ENO3C[3][3] = {{0.375, -1.25, 1.875},
+                                   {-0.125, 0.75, 0.375},
+                                   {0.375, 0.75, -0.125}};
+
+double tor_eno_pw(int k, const double* state) {
+  double dd[3][5];
+  double result = 0;
+  int r = 0;
+  for (int j = 0; j < 2 * k - 1; ++j) dd[0][j] = state[j];
+  for (int i = 1; i < k; ++i)
+    for (int j = 0; j < 2 * k - 1 - i; ++j) dd[i][j] = dd[i - 1][j + 1] - dd[i - 1][j];
+  int s = k - 1;
+  for (int i = 1; i < k; ++i)
+    if (fabs(dd[i][s - r]) > fabs(dd[i][s - r - 1])) r++;
+  for (int i = 0; i < k; ++i) {
+    if (k == 2)
+      result += ENO2C[s - r][i] * state[s - r + i];
+    else if (k == 3)
+      result += ENO3C[s - r][i] * state[s - r + i];
+  }
+  return result;
+}
+
+/* reconstruct_eno_pw: enflo_scalar/src_scalar/reconstruct.cc:80-104 */
+void tor_eno_rec(int k, const double* st, int ng, double* l, double* r) {
+  double comp[5] = {0, 0, 0, 0, 0};
+  int s = 2 * ng;
+  for (int j = 0; j < 2 * k - 1; ++j) comp[j] = st[s / 2 - k + j];
+  *l = tor_eno_pw(k, comp);
+  for (int j = 0; j < 2 * k - 1; ++j) comp[2 * k - 2 - j] = st[s / 2 - k + 1 + j];
+  *r = tor_eno_pw(k, comp);
+}
+
+/* ------------------------------------------------------------------------ */
+/* DSP-WENO network: enflo/src/neural_network.cc                             */
+int tor_net_load(const char* path, tor_net* net) {
+  /* load_network: neural_network.cc:82-130.  Blocks between BEGIN and END;
+   * rows of a weight block are output neurons; a "Biases" line switches to
+   * bias blocks. */
+  FILE* f = fopen(path, "r");
+  if (!f) return -1;
+  memset(net, 0, sizeof(*net));
+  char line[8192];
+  int weight = 1, nw = 0, nb = 0;
+  int rows = 0, cols = 0, in_block = 0, nbias = 0;
+  while (fgets(line, sizeof line, f)) {
+    size_t L = strlen(line);
+    while (L && (line[L - 1] == '\n' || line[L - 1] == '\r')) line[--L] = 0;
+    if (!in_block) {
+      if (strcmp(line, "BEGIN") == 0) {
+        in_block = 1;
+        rows = cols = nbias = 0;
+      } else if (strcmp(line, "Biases") == 0) {
+        weight = 0;
+      }
+      continue;
+    }
+    if (strcmp(line, "END") == 0) {
+      in_block = 0;
+      if (weight) {
+        if (nw >= TOR_MAX_LAYERS) { fclose(f); return -2; }
+        net->width[nw] = cols;     /* inputs of layer nw      */
+        net->width[nw + 1] = rows; /* outputs of layer nw     */
+        nw++;
+      } else {
+        if (nb >= TOR_MAX_LAYERS) { fclose(f); return -2; }
+        nb++;
+      }
+      continue;
+    }
+    char* p = line;
+    int c = 0;
+    for (;;) {
+      char* e;
+      double v = strtod(p, &e);
+      if (e == p) break;
+      p = e;
+      if (weight) {
+        if (rows >= TOR_MAX_WIDTH || c >= TOR_MAX_WIDTH) { fclose(f); return -2; }
+        net->W[nw][rows * TOR_MAX_WIDTH + c] = v;
+      } else {
+        if (nbias >= TOR_MAX_WIDTH) { fclose(f); return -2; }
+        net->b[nb][nbias++] = v;
+      }
+      c++;
+    }
+    if (weight) {
+      if (c > cols) cols = c;
+      rows++;
+    }
+  }
+  fclose(f);
+  net->n_layers = nw;
+  if (nw == 0 || nb != nw) return -3;
+  return 0;
+}
+
+int tor_net_save(const char* path, const tor_net* net) {
+  /* format written by save_net_weights, DSP-WENO_network_trainer.py:197-213 */
+  FILE* f = fopen(path, "w");
+  if (!f) return -1;
+  fprintf(f, "Weights\n");
+  for (int l = 0; l < net->n_layers; ++l) {
+    fprintf(f, "Layer %d\nBEGIN\n", l + 1);
+    for (int o = 0; o < net->width[l + 1]; ++o) {
+      for (int i = 0; i < net->width[l]; ++i)
+        fprintf(f, "%s%.18e", i ? " " : "", net->W[l][o * TOR_MAX_WIDTH + i]);
+      fprintf(f, "\n");
+    }
+    fprintf(f, "END\n");
+  }
+  fprintf(f, "Biases\n");
+  for (int l = 0; l < net->n_layers; ++l) {
+    fprintf(f, "Layer %d\nBEGIN\n", l + 1);
+    for (int o = 0; o < net->width[l + 1]; ++o) fprintf(f, "%.18e\n", net->b[l][o]);
+    fprintf(f, "END\n");
+  }
+  fclose(f);
+  return 0;
+}
+
+/* compute_vertices, vertex_mode 2, permutation 2: neural_network.cc:135-260 */
+void tor_nn_vertices(double theta_m, double theta_p, double max_delta,
+                     double xv[5], double yv[5]) {
+  const double thr = 1e-8;
+  double psi_p = (1 - theta_m) / (1 - theta_p);
+  double psi_m = 1 / psi_p;
+  double g1 = dmin(max_delta, 1.0 / 8.0);
+  double g2 = -dmin(max_delta, 3.0 / 8.0);
+  double x1 = 1.0 / 8.0 * (1 + psi_p) - g1 * psi_p;
+  double y1 = 1.0 / 8.0 * (1 + psi_m) - g2 * psi_m;
+  double x2 = 1.0 / 8.0 * (1 + psi_p) - g2 * psi_p;
+  double y2 = 1.0 / 8.0 * (1 + psi_m) - g1 * psi_m;
+  const double e = 1.0 / 8.0, t = -3.0 / 8.0, m = -1.0 / 8.0;
+#define SETV(a0, a1, a2, a3, a4, b0, b1, b2, b3, b4) \
+  do {                                               \
+    xv[0] = a0; xv[1] = a1; xv[2] = a2; xv[3] = a3; xv[4] = a4; \
+    yv[0] = b0; yv[1] = b1; yv[2] = b2; yv[3] = b3; yv[4] = b4; \
+  } while (0)
+  if ((theta_m - 1.0) > thr && (theta_p - 1.0) > thr) {
+    SETV(e, e, e, e, e, e, e, e, e, e);
+  } else if ((theta_m - 1.0) > thr && (theta_p - 1.0) < -1.0 * thr) {
+    if (psi_p < -1.0) {
+      if (x1 < g2) {
+        double nx = ((psi_m * psi_m) + psi_m) / (8 * ((psi_m * psi_m) + 1));
+        double ny = 1.0 / 8.0 * (1 + psi_m) - nx * psi_m;
+        SETV(nx, nx, nx, nx, nx, ny, ny, ny, ny, ny);
+      } else {
+        double cx = (2 * g2 + x1) / 3.0;
+        double cy = (2 * g1 + y2) / 3.0;
+        SETV(g2, x1, g2, cx, cx, g1, g1, y2, cy, cy);
+      }
+    } else {
+      if (y2 < g2)
+        SETV(g2, g1, g2, g1, 0.0, g1, g1, g2, g2, 0.0);
+      else
+        SETV(g2, g1, g2, g1, x2, g1, g1, g2, y2, g2);
+    }
+  } else if ((theta_m - 1.0) < -1.0 * thr && (theta_p - 1.0) > thr) {
+    if (psi_p < -1.0) {
+      if (x1 < g2)
+        SETV(g2, g1, g2, g1, 0.0, g1, g1, g2, g2, 0.0);
+      else
+        SETV(g1, g1, g2, x1, g2, g2, g1, g2, g1, y1);
+    } else {
+      if (y2 < g2) {
+        double nx = ((psi_m * psi_m) + psi_m) / (8 * ((psi_m * psi_m) + 1));
+        double ny = 1.0 / 8.0 * (1 + psi_m) - nx * psi_m;
+        SETV(nx, nx, nx, nx, nx, ny, ny, ny, ny, ny);
+      } else {
+        double cx = (2 * g1 + x1) / 3.0;
+        double cy = (2 * g2 + y2) / 3.0;
+        SETV(g1, x1, g1, cx, cx, g2, g2, y2, cy, cy);
+      }
+    }
+  } else if (fabs(theta_m - 1.0) < thr && (theta_p - 1.0) > thr) {
+    SETV(e, e, e, e, e, t, e, e, t, m);
+  } else if (fabs(theta_p - 1.0) < thr && (theta_m - 1.0) > thr) {
+    SETV(e, e, t, t, m, e, e, e, e, e);
+  } else {
+    SETV(e, e, t, t, m, e, t, t, e, m);
+  }
+#undef SETV
+}
+
+/* scale + features: neural_network.cc:22-38, 300-313 */
+static void nn_prep(const double in[4], double st[5], double* tm_out,
+                    double* tp_out, double* maxd_out) {
+  double c[4];
+  for (int i = 0; i < 4; ++i) c[i] = in[i];
+  double mv = 1.0;
+  for (int i = 0; i < 4; ++i) mv = dmax(mv, fabs(c[i]));
+  for (int i = 0; i < 4; ++i) c[i] /= mv;
+  double tm = (c[3] - c[2]) / (c[2] - c[1]);
+  double tp = (c[1] - c[0]) / (c[2] - c[1]);
+  double d1 = fabs(c[1] - c[0]);
+  double d2 = fabs(c[2] - c[1]);
+  double d3 = fabs(c[3] - c[2]);
+  double md = dmax(dmax(d1, d2), d3);
+  st[0] = tanh(tm);
+  st[1] = tanh(tp);
+  st[2] = d1;
+  st[3] = d2;
+  st[4] = d3;
+  *tm_out = tm;
+  *tp_out = tp;
+  *maxd_out = md;
+}
+
+void tor_nn_features(const double u[4], double feat[5], double xv[5],
+                     double yv[5]) {
+  double tm, tp, md;
+  nn_prep(u, feat, &tm, &tp, &md);
+  tor_nn_vertices(tm, tp, md, xv, yv);
+}
+
+/* forward: neural_network.cc:16-77 (+ relu 265-271, softmax 276-295,
+ * compute_recon 320-325).  The state vector has 5 entries; a layer writes its
+ * first width[l+1] entries and leaves the rest (the reference's behaviour for
+ * widths <= 5). */
+void tor_nn_forward(const tor_net* net, const double in[4], double* recl,
+                    double* recr) {
+  double st[TOR_MAX_WIDTH];
+  double tm, tp, md, xv[5], yv[5];
+  for (int i = 0; i < TOR_MAX_WIDTH; ++i) st[i] = 0.0;
+  nn_prep(in, st, &tm, &tp, &md);
+  tor_nn_vertices(tm, tp, md, xv, yv);
+  for (int l = 0; l < net->n_layers; ++l) {
+    double tmp[TOR_MAX_WIDTH];
+    int no = net->width[l + 1], ni = net->width[l];
+    for (int j = 0; j < no; ++j) {
+      double dot = 0.0;
+      for (int k = 0; k < ni; ++k) dot += net->W[l][j * TOR_MAX_WIDTH + k] * st[k];
+      tmp[j] = dot;
+    }
+    for (int j = 0; j < no; ++j) st[j] = tmp[j] + net->b[l][j];
+    if (l != net->n_layers - 1) {
+      int nr = no > 5 ? no : 5; /* relu over the state vector (>= 5 entries) */
+      for (int j = 0; j < nr; ++j) st[j] = dmax(0.0, st[j]);
+    }
+  }
+  double mx = 0.0;
+  for (int i = 0; i < 5; ++i) mx = dmax(mx, st[i]);
+  double sum = 0;
+  for (int i = 0; i < 5; ++i) {
+    st[i] = exp(st[i] - mx);
+    sum += st[i];
+  }
+  for (int i = 0; i < 5; ++i) st[i] /= sum;
+  double C1 = 0.0, C2 = 0.0;
+  for (int i = 0; i < 5; ++i) {
+    C1 += st[i] * xv[i];
+    C2 += st[i] * yv[i];
+  }
+  double w1 = 1.0 / 4.0 - 2 * C1;
+  double tw0 = 1.0 / 4.0 - 2 * C2;
+  double wt[4] = {1.0 - w1, w1, tw0, 1.0 - tw0};
+  *recl = wt[0] * 0.5 * (in[1] + in[2]) + wt[1] * (-0.5 * in[0] + 1.5 * in[1]);
+  *recr = wt[3] * 0.5 * (in[1] + in[2]) + wt[2] * (-0.5 * in[3] + 1.5 * in[2]);
+}
+
+/* reconstruct dispatch: enflo/src/reconstruct.cc:115-157, 192-239 */
+void tor_reconstruct(const tecno_cfg* cfg, const tor_net* net,
+                     const double* st, int ng, double* l, double* r) {
+  int s = 2 * ng;
+  switch (cfg->recon) {
+    case TECNO_REC_SP_WENO:
+      tor_sp_weno(st + s / 2 - 2, cfg->spweno_corr, l, r);
+      break;
+    case TECNO_REC_ENO:
+      tor_eno_rec(cfg->eno_level, st, ng, l, r);
+      break;
+    case TECNO_REC_SP_WENO_DL: {
+      const double* c = st + s / 2 - 2;
+      if (fabs(c[2] - c[1]) < 1e-15) {
+        *l = 0.0;
+        *r = 0.0;
+      } else {
+        tor_nn_forward(net, c, l, r);
+      }
+      break;
+    }
+    case TECNO_REC_FIRST:
+    default:
+      *l = st[s / 2 - 1];
+      *r = st[s / 2];
+      break;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* scalar fluxes: enflo_scalar/src_scalar/{lin,bur}_*.cc, num_flux.cc:12-57  */
+typedef struct {
+  double h[3];
+  double area[3];
+} geom_t;
+
+static double sc_dot_axis(const double v[3], int axis) {
+  /* Vector::operator* with a unit axis normal (vec.h:239-242) */
+  double n[3] = {0.0, 0.0, 0.0};
+  n[axis] = 1.0;
+  return n[0] * v[0] + n[1] * v[1] + n[2] * v[2];
+}
+
+static double lin_ec(const tecno_cfg* c, double ul, double ur, int axis) {
+  double vel = sc_dot_axis(c->vel, axis);
+  return 0.5 * vel * (ul + ur); /* lin_ec.cc:10-20 */
+}
+static double bur_ec(double ul, double ur) {
+  return (ul * ul + ur * ur + ul * ur) / 6.0; /* bur_ec.cc:10-19 */
+}
+
+static double scalar_face_flux(const tecno_cfg* c, const tor_net* net,
+                               const double* S, int ng, int axis,
+                               const geom_t* g) {
+  int s = 2 * ng;
+  double ul = S[s / 2 - 1], ur = S[s / 2];
+  double flux = 0.0, f0;
+  switch (c->flux) {
+    case TECNO_FLUX_LIN_EC:
+    case TECNO_FLUX_LIN_TECNO:
+      flux = lin_ec(c, ul, ur, axis);
+      break;
+    case TECNO_FLUX_LIN_EC4:
+    case TECNO_FLUX_LIN_TECNO4:
+      /* lin_ec4.cc:10-34 — note flux0*4.0/3.0 associates as (flux0*4)/3 */
+      flux = 0.0;
+      f0 = lin_ec(c, S[s / 2 - 1], S[s / 2], axis);
+      flux += f0 * 4.0 / 3.0;
+      f0 = lin_ec(c, S[s / 2 - 2], S[s / 2], axis);
+      flux += f0 * (-1.0 / 6.0);
+      f0 = lin_ec(c, S[s / 2 - 1], S[s / 2 + 1], axis);
+      flux += f0 * (-1.0 / 6.0);
+      break;
+    case TECNO_FLUX_BUR_EC:
+    case TECNO_FLUX_BUR_TECNO:
+      flux = bur_ec(ul, ur);
+      break;
+    case TECNO_FLUX_BUR_EC4:
+    case TECNO_FLUX_BUR_TECNO4:
+      /* bur_ec4.cc:10-34 */
+      flux = 0.0;
+      f0 = bur_ec(S[s / 2 - 1], S[s / 2]);
+      flux += f0 * (4.0 / 3.0);
+      f0 = bur_ec(S[s / 2 - 2], S[s / 2]);
+      flux += f0 * (-1.0 / 6.0);
+      f0 = bur_ec(S[s / 2 - 1], S[s / 2 + 1]);
+      flux += f0 * (-1.0 / 6.0);
+      break;
+  }
+  int tecno = (c->flux == TECNO_FLUX_LIN_TECNO || c->flux == TECNO_FLUX_LIN_TECNO4 ||
+               c->flux == TECNO_FLUX_BUR_TECNO || c->flux == TECNO_FLUX_BUR_TECNO4);
+  if (tecno) {
+    /* lin_tecno4.cc:10-31, bur_tecno4.cc:10-35 */
+    double D;
+    if (c->flux == TECNO_FLUX_LIN_TECNO || c->flux == TECNO_FLUX_LIN_TECNO4)
+      D = fabs(sc_dot_axis(c->vel, axis));
+    else
+      D = 0.5 * (fabs(ul) + fabs(ur));
+    double uij, uji;
+    tor_reconstruct(c, net, S, ng, &uij, &uji);
+    flux -= 0.5 * D * (uji - uij);
+  }
+  return flux * g->area[axis]; /* num_flux.cc:53 */
+}
+
+/* ------------------------------------------------------------------------ */
+/* Euler fluxes                                                             */
+typedef struct { double rho, u[3], p; } prim_t;
+typedef struct { double f, v[3], l; } ent_t;
+
+static double vdot(const double a[3], const double b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static double vsq(const double a[3]) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2]; }
+
+/* kepec_flux: enflo/src/kepec.cc:10-48 (area included) */
+static void kepec_flux(const tecno_cfg* c, const prim_t* Pl, const prim_t* Pr,
+                       const double n[3], double area, double F[5]) {
+  double R = c->gas_const, G = c->gamma;
+  double rhol = Pl->rho, rhor = Pr->rho;
+  double rho = tor_logavg(rhol, rhor);
+  double templ = Pl->p / (R * rhol);
+  double tempr = Pr->p / (R * rhor);
+  double vel2 = 0.5 * (vsq(Pl->u) + vsq(Pr->u));
+  double betal = 0.5 / (R * templ);
+  double betar = 0.5 / (R * tempr);
+  double beta = tor_logavg(betal, betar);
+  double p = 0.5 * (rhol + rhor) / (betal + betar);
+  double vel[3];
+  for (int d = 0; d < 3; ++d) vel[d] = Pl->u[d] * 0.5 + Pr->u[d] * 0.5;
+  double vn = vdot(vel, n);
+  double mass = rho * vn;
+  double mom[3];
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * p + vel[d] * mass;
+  double en = 0.5 * (1.0 / ((G - 1.0) * beta) - vel2) * mass + vdot(mom, vel);
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+/* roe_ec_flux: enflo/src/roe_ec.cc:10-69 */
+static void roe_ec_flux(const tecno_cfg* c, const prim_t* Pl, const prim_t* Pr,
+                        const double n[3], double area, double F[5]) {
+  double G = c->gamma;
+  double rhol = Pl->rho, rhor = Pr->rho, pl = Pl->p, pr = Pr->p;
+  double bl = sqrt(rhol / pl);
+  double br = sqrt(rhor / pr);
+  double z1b = 0.5 * (bl + br);
+  double zvb[3];
+  for (int d = 0; d < 3; ++d) zvb[d] = Pl->u[d] * (0.5 * bl) + Pr->u[d] * (0.5 * br);
+  double z5b = 0.5 * (bl * pl + br * pr);
+  double zvn = vdot(zvb, n);
+  double z1l = tor_logavg(bl, br);
+  double z5l = tor_logavg(bl * pl, br * pr);
+  double mass = z5l * zvn;
+  double mom[3];
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * (z5b / z1b) + zvb[d] * (mass / z1b);
+  double en = 0.5 * ((G + 1.0) / (G - 1.0) * mass / z1l + vdot(mom, zvb)) / z1b;
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+/* kep_flux: enflo/src/kep.cc:10-44, Enthalpy fv.cc:565-569 */
+static double enthalpy(const tecno_cfg* c, const prim_t* P) {
+  double a = sqrt(c->gamma * P->p / P->rho);
+  return a * a / (c->gamma - 1.0) + 0.5 * vsq(P->u);
+}
+static void kep_flux(const tecno_cfg* c, const prim_t* Pl, const prim_t* Pr,
+                     const double n[3], double area, double F[5]) {
+  double rho = 0.5 * (Pl->rho + Pr->rho);
+  double p = 0.5 * (Pl->p + Pr->p);
+  double vel[3];
+  for (int d = 0; d < 3; ++d) vel[d] = Pl->u[d] * 0.5 + Pr->u[d] * 0.5;
+  double H = 0.5 * (enthalpy(c, Pl) + enthalpy(c, Pr));
+  double vn = vdot(vel, n);
+  double mass = rho * vn;
+  double mom[3];
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * p + vel[d] * mass;
+  double en = rho * H * vn;
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+typedef void (*ec_fn)(const tecno_cfg*, const prim_t*, const prim_t*,
+                      const double*, double, double*);
+
+/* 4th-order combination: kepec4.cc:10-38, roe_ec4.cc:10-34 (Flux::sadd) */
+static void ec4_flux(ec_fn fn, const tecno_cfg* c, const prim_t* P, int ng,
+                     const double n[3], double area, double F[5]) {
+  int s = 2 * ng;
+  double f0[5];
+  for (int v = 0; v < 5; ++v) F[v] = 0.0;
+  fn(c, &P[s / 2 - 1], &P[s / 2], n, area, f0);
+  for (int v = 0; v < 5; ++v) F[v] += f0[v] * (4.0 / 3.0);
+  fn(c, &P[s / 2 - 2], &P[s / 2], n, area, f0);
+  for (int v = 0; v < 5; ++v) F[v] += f0[v] * (-1.0 / 6.0);
+  fn(c, &P[s / 2 - 1], &P[s / 2 + 1], n, area, f0);
+  for (int v = 0; v < 5; ++v) F[v] += f0[v] * (-1.0 / 6.0);
+}
+
+/* TeCNO fluxes: kepes_tecno4_roe.cc:10-176 and its six variants
+ * (kepes_tecno_roe.cc, kepes_tecno{,4}_rusanov.cc, roe_tecno{,4}_roe.cc,
+ * kep_tecno_roe.cc), which differ only in the entropy-conservative flux and
+ * in the eigenvalue matrix Lambda. */
+static void euler_face_flux(const tecno_cfg* c, const tor_net* net,
+                            const prim_t* P, const ent_t* V, int ng, int axis,
+                            const geom_t* g, double F[5]) {
+  double n[3] = {0.0, 0.0, 0.0};
+  n[axis] = 1.0;
+  double area = g->area[axis];
+  int s = 2 * ng;
+  int fl = c->flux;
+  switch (fl) {
+    case TECNO_FLUX_KEPEC: kepec_flux(c, &P[s / 2 - 1], &P[s / 2], n, area, F); return;
+    case TECNO_FLUX_KEPEC4: ec4_flux(kepec_flux, c, P, ng, n, area, F); return;
+    case TECNO_FLUX_ROE_EC: roe_ec_flux(c, &P[s / 2 - 1], &P[s / 2], n, area, F); return;
+    case TECNO_FLUX_ROE_EC4: ec4_flux(roe_ec_flux, c, P, ng, n, area, F); return;
+    case TECNO_FLUX_KEP: kep_flux(c, &P[s / 2 - 1], &P[s / 2], n, area, F); return;
+    default: break;
+  }
+  double R = c->gas_const, G = c->gamma;
+  const prim_t* Pl = &P[s / 2 - 1];
+  const prim_t* Pr = &P[s / 2];
+  double rhol = Pl->rho, rhor = Pr->rho;
+  double rho = tor_logavg(rhol, rhor);
+  double templ = Pl->p / (R * rhol);
+  double tempr = Pr->p / (R * rhor);
+  double betal = 0.5 / (R * templ);
+  double betar = 0.5 / (R * tempr);
+  double beta = tor_logavg(betal, betar);
+  double p = 0.5 * (rhol + rhor) / (betal + betar);
+  double vel[3];
+  for (int d = 0; d < 3; ++d) vel[d] = Pl->u[d] * 0.5 + Pr->u[d] * 0.5;
+  double vn = vdot(vel, n);
+
+  switch (fl) {
+    case TECNO_FLUX_KEPES_TECNO4_ROE:
+    case TECNO_FLUX_KEPES_TECNO4_RUSANOV:
+      ec4_flux(kepec_flux, c, P, ng, n, area, F);
+      break;
+    case TECNO_FLUX_KEPES_TECNO_ROE:
+    case TECNO_FLUX_KEPES_TECNO_RUSANOV:
+      kepec_flux(c, Pl, Pr, n, area, F);
+      break;
+    case TECNO_FLUX_ROE_TECNO4_ROE:
+      ec4_flux(roe_ec_flux, c, P, ng, n, area, F);
+      break;
+    case TECNO_FLUX_ROE_TECNO_ROE:
+      roe_ec_flux(c, Pl, Pr, n, area, F);
+      break;
+    case TECNO_FLUX_KEP_TECNO_ROE:
+      kep_flux(c, Pl, Pr, n, area, F);
+      break;
+  }
+  double inv_area = 1 / area;
+  for (int v = 0; v < 5; ++v) F[v] *= inv_area;
+
+  double a = sqrt(0.5 * G / beta);
+  double H = a * a / (G - 1.0) + 0.5 * vsq(vel);
+  double v1 = vel[0] * n[1] - vel[1] * n[0];
+  double v2 = vel[2] * n[0] - vel[0] * n[2];
+  double Rm[5][5] = {
+      {1, 1, 0, 0, 1},
+      {vel[0] - a * n[0], vel[0], n[1], -n[2], vel[0] + a * n[0]},
+      {vel[1] - a * n[1], vel[1], -n[0], 0, vel[1] + a * n[1]},
+      {vel[2] - a * n[2], vel[2], 0, n[0], vel[2] + a * n[2]},
+      {H - a * vn, 0.5 * vsq(vel), v1, v2, H + a * vn}};
+  double const1 = sqrt(rho / G);
+  double psqrt = sqrt(p);
+  double S[5] = {sqrt(0.5) * const1, sqrt(G - 1.0) * const1, psqrt, psqrt,
+                 sqrt(0.5) * const1};
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) Rm[j][i] *= S[i];
+
+  double Lam[5];
+  int rusanov = (fl == TECNO_FLUX_KEPES_TECNO_RUSANOV || fl == TECNO_FLUX_KEPES_TECNO4_RUSANOV);
+  if (rusanov) {
+    double lr = fabs(vn) + fabs(a);
+    for (int i = 0; i < 5; ++i) Lam[i] = lr;
+  } else {
+    double vnl = vdot(Pl->u, n);
+    double vnr = vdot(Pr->u, n);
+    double al = sqrt(0.5 * G / betal);
+    double ar = sqrt(0.5 * G / betar);
+    double LL0 = vnl - al, LR0 = vnr - ar, LL3 = vnl, LR3 = vnr;
+    double l2 = fabs(vn);
+    double eb = 0.0, ea = 0.0;
+    if (c->ec1_fix && fl != TECNO_FLUX_KEP_TECNO_ROE) ea = 1.0 / 6.0;
+    Lam[0] = (1 + eb) * fabs(vn - a) + ea * fabs(LL0 - LR0);
+    Lam[1] = l2;
+    Lam[2] = l2;
+    Lam[3] = l2;
+    Lam[4] = (1 + eb) * fabs(vn + a) + ea * fabs(LL3 - LR3);
+  }
+
+  /* scaled entropy variables Z[k] = R~^T V[k] (kepes_tecno4_roe.cc:125-137) */
+  double Z[6][5];
+  for (int k = 0; k < s; ++k)
+    for (int i = 0; i < 5; ++i)
+      Z[k][i] = 0.0 + (Rm[0][i] * V[k].f + Rm[1][i] * V[k].v[0] + Rm[2][i] * V[k].v[1] +
+                       Rm[3][i] * V[k].v[2] + Rm[4][i] * V[k].l);
+  double Zl[5], Zr[5];
+  for (int i = 0; i < 5; ++i) {
+    double st[6];
+    for (int k = 0; k < s; ++k) st[k] = Z[k][i];
+    tor_reconstruct(c, net, st, ng, &Zl[i], &Zr[i]);
+  }
+  double dZ[5], Diff[5];
+  for (int i = 0; i < 5; ++i) {
+    dZ[i] = Zr[i] - Zl[i];
+    Diff[i] = 0;
+  }
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) Diff[i] += Rm[i][j] * Lam[j] * dZ[j];
+  F[0] -= 0.5 * Diff[0];
+  F[1] -= 0.5 * Diff[1];
+  F[2] -= 0.5 * Diff[2];
+  if (c->fix_zmom)
+    F[3] -= 0.5 * Diff[3];
+  else
+    F[2] -= 0.5 * Diff[3]; /* kepes_tecno4_roe.cc:170 */
+  F[4] -= 0.5 * Diff[4];
+  for (int v = 0; v < 5; ++v) F[v] *= area;
+}
+
+/* prim_to_ent: enflo/src/fv.cc:421-431, Entropy 557-560 */
+static void prim_to_ent(double G, const prim_t* P, ent_t* V) {
+  double beta = P->rho / (2 * P->p);
+  double s = log(P->p) - G * log(P->rho);
+  V->f = (G - s) / (G - 1) - vsq(P->u) * beta;
+  for (int d = 0; d < 3; ++d) V->v[d] = P->u[d] * (2 * beta);
+  V->l = -2 * beta;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batched wrappers                                                          */
+static void make_geom(const double h[3], geom_t* g) {
+  for (int d = 0; d < 3; ++d) g->h[d] = h[d];
+  g->area[0] = h[1] * h[2];
+  g->area[1] = h[0] * h[2];
+  g->area[2] = h[0] * h[1];
+}
+
+void tor_sp_weno_batch(int corr, const double* u4, int64_t n, double* l, double* r) {
+  for (int64_t i = 0; i < n; ++i) tor_sp_weno(u4 + 4 * i, corr, l + i, r + i);
+}
+void tor_reconstruct_batch(const tecno_cfg* cfg, const tor_net* net,
+                           const double* st, int ng, int64_t n, double* l, double* r) {
+  for (int64_t i = 0; i < n; ++i) tor_reconstruct(cfg, net, st + 2 * ng * i, ng, l + i, r + i);
+}
+void tor_nn_forward_batch(const tor_net* net, const double* u4, int64_t n,
+                          double* l, double* r) {
+  for (int64_t i = 0; i < n; ++i) tor_nn_forward(net, u4 + 4 * i, l + i, r + i);
+}
+void tor_logavg_batch(const double* a, const double* b, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = tor_logavg(a[i], b[i]);
+}
+void tor_prim_to_ent_batch(const tecno_cfg* cfg, const double* P, int64_t n, double* V) {
+  for (int64_t i = 0; i < n; ++i) {
+    prim_t p = {P[5 * i], {P[5 * i + 1], P[5 * i + 2], P[5 * i + 3]}, P[5 * i + 4]};
+    ent_t v;
+    prim_to_ent(cfg->gamma, &p, &v);
+    V[5 * i] = v.f;
+    V[5 * i + 1] = v.v[0];
+    V[5 * i + 2] = v.v[1];
+    V[5 * i + 3] = v.v[2];
+    V[5 * i + 4] = v.l;
+  }
+}
+void tor_flux_batch(const tecno_cfg* cfg, const tor_net* net, const double* P,
+                    const double* V, int ng, int64_t n, int axis,
+                    const double h[3], double* F) {
+  geom_t g;
+  make_geom(h, &g);
+  int s = 2 * ng;
+  for (int64_t i = 0; i < n; ++i) {
+    if (cfg->system == TECNO_SYS_SCALAR) {
+      F[i] = scalar_face_flux(cfg, net, P + s * i, ng, axis, &g);
+    } else {
+      prim_t pp[6];
+      ent_t vv[6];
+      for (int k = 0; k < s; ++k) {
+        const double* q = P + (i * s + k) * 5;
+        const double* w = V + (i * s + k) * 5;
+        pp[k].rho = q[0];
+        pp[k].u[0] = q[1];
+        pp[k].u[1] = q[2];
+        pp[k].u[2] = q[3];
+        pp[k].p = q[4];
+        vv[k].f = w[0];
+        vv[k].v[0] = w[1];
+        vv[k].v[1] = w[2];
+        vv[k].v[2] = w[3];
+        vv[k].l = w[4];
+      }
+      euler_face_flux(cfg, net, pp, vv, ng, axis, &g, F + 5 * i);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* whole-grid solver (FV::run and friends)                                   */
+struct tor_solver {
+  tecno_cfg c;
+  tor_net net;
+  int has_net;
+  int ng;
+  int n[3];       /* cells per direction held */
+  int active[3];  /* direction swept          */
+  int N[3];       /* padded extents            */
+  geom_t g;
+  double hmin;
+  int64_t ncell, npad;
+  int nvar;       /* 1 scalar, 5 euler (always 3 velocity comps internally) */
+  /* scalar: u (padded) ; euler: prim (padded, AoS prim_t), ent (padded),
+   * U, Uold, res (interior, 5 comps) */
+  double* u;
+  double* uold;
+  prim_t* P;
+  ent_t* V;
+  double* U;
+  double* Uo;
+  double* res;
+  double* Fface;  /* face flux scratch, (ncell + extra) x nvar per axis */
+  double time;
+  int iter;
+  int first_stage_prim; /* euler: prim holds IC values (not con_to_prim)   */
+};
+
+static int64_t pidx(const tor_solver* s, int i, int j, int k) {
+  /* padded index of interior-relative coordinates (may be negative) */
+  return ((int64_t)(k + s->ng) * s->N[1] + (j + s->ng)) * s->N[0] + (i + s->ng);
+}
+static int64_t iidx(const tor_solver* s, int i, int j, int k) {
+  return ((int64_t)k * s->n[1] + j) * s->n[0] + i;
+}
+
+static int flux_ng(int flux) {
+  switch (flux) {
+    case TECNO_FLUX_LIN_EC4: case TECNO_FLUX_LIN_TECNO4: case TECNO_FLUX_BUR_EC4:
+    case TECNO_FLUX_BUR_TECNO4: case TECNO_FLUX_KEPES_TECNO4_ROE:
+    case TECNO_FLUX_KEPES_TECNO4_RUSANOV: case TECNO_FLUX_ROE_TECNO4_ROE:
+    case TECNO_FLUX_KEPEC4: case TECNO_FLUX_ROE_EC4:
+      return 2;
+    default:
+      return 1;
+  }
+}
+static int rec_ng(const tecno_cfg* c) {
+  switch (c->recon) {
+    case TECNO_REC_ENO: return c->eno_level;
+    case TECNO_REC_FIRST: return 1;
+    default: return 2;
+  }
+}
+
+void tor_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+tor_solver* tor_create(const tecno_cfg* cfg, const tor_net* net, int faithful) {
+  tor_solver* s = (tor_solver*)calloc(1, sizeof(tor_solver));
+  if (!s) return NULL;
+  s->c = *cfg;
+  if (net) {
+    s->net = *net;
+    s->has_net = 1;
+  }
+  s->ng = flux_ng(cfg->flux);
+  if (rec_ng(cfg) > s->ng) s->ng = rec_ng(cfg);
+  for (int d = 0; d < 3; ++d) {
+    double h = (cfg->hi[d] - cfg->lo[d]) / cfg->n[d];
+    s->g.h[d] = h;
+    int act = faithful ? 1 : (d < cfg->dims);
+    s->active[d] = act;
+    s->n[d] = act ? cfg->n[d] : 1;
+    s->N[d] = s->n[d] + 2 * s->ng;
+  }
+  make_geom(s->g.h, &s->g);
+  s->hmin = dmin(s->g.h[0], dmin(s->g.h[1], s->g.h[2]));
+  s->ncell = (int64_t)s->n[0] * s->n[1] * s->n[2];
+  s->npad = (int64_t)s->N[0] * s->N[1] * s->N[2];
+  s->nvar = (cfg->system == TECNO_SYS_SCALAR) ? 1 : 5;
+  int64_t nf = s->ncell + (int64_t)s->n[1] * s->n[2] + (int64_t)s->n[0] * s->n[2] +
+               (int64_t)s->n[0] * s->n[1] + 16;
+  s->Fface = (double*)calloc(nf * 3 * s->nvar, sizeof(double));
+  if (s->nvar == 1) {
+    s->u = (double*)calloc(s->npad, sizeof(double));
+    s->uold = (double*)calloc(s->npad, sizeof(double));
+  } else {
+    s->P = (prim_t*)calloc(s->npad, sizeof(prim_t));
+    s->V = (ent_t*)calloc(s->npad, sizeof(ent_t));
+    s->U = (double*)calloc(s->ncell * 5, sizeof(double));
+    s->Uo = (double*)calloc(s->ncell * 5, sizeof(double));
+  }
+  s->res = (double*)calloc(s->ncell * s->nvar, sizeof(double));
+  return s;
+}
+
+void tor_destroy(tor_solver* s) {
+  if (!s) return;
+  free(s->u);
+  free(s->uold);
+  free(s->P);
+  free(s->V);
+  free(s->U);
+  free(s->Uo);
+  free(s->res);
+  free(s->Fface);
+  free(s);
+}
+
+int64_t tor_num_cells(const tor_solver* s) { return s->ncell; }
+int tor_num_vars(const tor_solver* s) { return s->nvar; }
+
+/* ghost fill.  scalar: enflo_scalar fv.cc:110-281; euler: enflo fv.cc:121-361
+ * (primitive variables; edges/corners are only needed by the viscous terms and
+ * are filled periodically as the reference does). */
+static void scalar_bc(tor_solver* s, double* u) {
+  int g = s->ng;
+  for (int d = 0; d < 3; ++d) {
+    int nd = s->n[d];
+    int e1 = (d == 0) ? 1 : 0, e2 = (d == 2) ? 1 : 2;
+    int n1 = s->n[e1], n2 = s->n[e2];
+    for (int side = 0; side < 2; ++side) {
+      int bc = s->c.bc[2 * d + side];
+      for (int a = 0; a < n1; ++a)
+        for (int b = 0; b < n2; ++b)
+          for (int l = 0; l < g; ++l) {
+            int gi = side ? nd + l : l - g; /* ghost coordinate */
+            int src;
+            double sign = 1.0;
+            if (bc == TECNO_BC_PERIODIC)
+              src = side ? l : nd - g + l;
+            else if (bc == TECNO_BC_WALL) {
+              src = side ? nd - 1 - l : g - 1 - l;
+              sign = -1.0;
+            } else
+              src = side ? nd - 1 : 0;
+            int cg[3], cs[3];
+            cg[d] = gi;
+            cs[d] = src;
+            cg[e1] = cs[e1] = a;
+            cg[e2] = cs[e2] = b;
+            double v = u[pidx(s, cs[0], cs[1], cs[2])];
+            u[pidx(s, cg[0], cg[1], cg[2])] = (sign < 0) ? -v : v;
+          }
+    }
+  }
+}
+
+static void euler_bc(tor_solver* s) {
+  int g = s->ng;
+  for (int d = 0; d < 3; ++d) {
+    int nd = s->n[d];
+    int e1 = (d == 0) ? 1 : 0, e2 = (d == 2) ? 1 : 2;
+    int n1 = s->n[e1], n2 = s->n[e2];
+    for (int side = 0; side < 2; ++side) {
+      int bc = s->c.bc[2 * d + side];
+      for (int a = 0; a < n1; ++a)
+        for (int b = 0; b < n2; ++b)
+          for (int l = 0; l < g; ++l) {
+            int gi = side ? nd + l : l - g;
+            int src, flip = 0;
+            if (bc == TECNO_BC_PERIODIC)
+              src = side ? l : nd - g + l;
+            else if (bc == TECNO_BC_WALL) {
+              src = side ? nd - 1 - l : g - 1 - l;
+              flip = 1;
+            } else
+              src = side ? nd - 1 : 0;
+            int cg[3], cs[3];
+            cg[d] = gi;
+            cs[d] = src;
+            cg[e1] = cs[e1] = a;
+            cg[e2] = cs[e2] = b;
+            prim_t p = s->P[pidx(s, cs[0], cs[1], cs[2])];
+            if (flip) p.u[d] = -p.u[d];
+            s->P[pidx(s, cg[0], cg[1], cg[2])] = p;
+          }
+    }
+  }
+}
+
+int tor_set_prim(tor_solver* s, const double* prim) {
+  int64_t N = s->ncell;
+  for (int k = 0; k < s->n[2]; ++k)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int i = 0; i < s->n[0]; ++i) {
+        int64_t q = iidx(s, i, j, k), p = pidx(s, i, j, k);
+        if (s->nvar == 1) {
+          s->u[p] = prim[q];
+        } else {
+          prim_t* P = &s->P[p];
+          P->rho = prim[q];
+          P->u[0] = prim[N + q];
+          P->u[1] = prim[2 * N + q];
+          P->u[2] = prim[3 * N + q];
+          P->p = prim[4 * N + q];
+          /* prim_to_con: fv.cc:405-416 */
+          double G = s->c.gamma;
+          double* U = s->U + 5 * q;
+          U[0] = P->rho;
+          for (int d = 0; d < 3; ++d) U[1 + d] = P->u[d] * P->rho;
+          U[4] = 0.5 * P->rho * vsq(P->u) + P->p / (G - 1.0);
+        }
+      }
+  if (s->nvar == 1)
+    scalar_bc(s, s->u);
+  else
+    euler_bc(s);
+  s->time = 0.0;
+  s->iter = 0;
+  return 0;
+}
+
+int tor_get_prim(const tor_solver* s, double* prim) {
+  int64_t N = s->ncell;
+  for (int k = 0; k < s->n[2]; ++k)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int i = 0; i < s->n[0]; ++i) {
+        int64_t q = iidx(s, i, j, k), p = pidx(s, i, j, k);
+        if (s->nvar == 1) {
+          prim[q] = s->u[p];
+        } else {
+          const prim_t* P = &s->P[p];
+          prim[q] = P->rho;
+          prim[N + q] = P->u[0];
+          prim[2 * N + q] = P->u[1];
+          prim[3 * N + q] = P->u[2];
+          prim[4 * N + q] = P->p;
+        }
+      }
+  return 0;
+}
+
+/* compute_dt: enflo fv.cc:366-383, enflo_scalar fv.cc:286-311.  The minimum
+ * of hmin/speed_i is accumulated in the reference's cell order. */
+static double compute_dt(tor_solver* s) {
+  double dt = 1.0e20;
+  for (int i = 0; i < s->n[0]; ++i)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int k = 0; k < s->n[2]; ++k) {
+        double speed;
+        if (s->nvar == 1) {
+          if (s->c.model == TECNO_MODEL_LINADV)
+            speed = sqrt(s->c.vel[0] * s->c.vel[0] + s->c.vel[1] * s->c.vel[1] +
+                         s->c.vel[2] * s->c.vel[2]);
+          else
+            speed = fabs(s->u[pidx(s, i, j, k)]);
+        } else {
+          const prim_t* P = &s->P[pidx(s, i, j, k)];
+          speed = sqrt(vsq(P->u)) + sqrt(s->c.gamma * P->p / P->rho);
+        }
+        dt = dmin(dt, s->hmin / speed);
+      }
+  return dt * s->c.cfl;
+}
+
+/* one face stencil along axis d ending at padded offset; gathers 2ng cells */
+static void scalar_residual(tor_solver* s) {
+  int g = s->ng;
+  int64_t N = s->ncell;
+  for (int64_t q = 0; q < N; ++q) s->res[q] = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    if (!s->active[d]) continue;
+    int fn[3] = {s->n[0], s->n[1], s->n[2]};
+    fn[d] += 1;
+    int64_t nf = (int64_t)fn[0] * fn[1] * fn[2];
+    double* F = s->Fface;
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < nf; ++f) {
+      int i = (int)(f % fn[0]);
+      int j = (int)((f / fn[0]) % fn[1]);
+      int k = (int)(f / ((int64_t)fn[0] * fn[1]));
+      double st[6];
+      for (int l = 0; l < 2 * g; ++l) {
+        int c[3] = {i, j, k};
+        c[d] += l - g; /* face f-1/2 of cell c: cells c-g .. c+g-1 */
+        st[l] = s->u[pidx(s, c[0], c[1], c[2])];
+      }
+      F[f] = scalar_face_flux(&s->c, &s->net, st, g, d, &s->g);
+    }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->n[2]; ++k)
+      for (int j = 0; j < s->n[1]; ++j)
+        for (int i = 0; i < s->n[0]; ++i) {
+          int c[3] = {i, j, k};
+          int64_t fl = ((int64_t)c[2] * fn[1] + c[1]) * fn[0] + c[0];
+          c[d] += 1;
+          int64_t fr = ((int64_t)c[2] * fn[1] + c[1]) * fn[0] + c[0];
+          double* r = &s->res[iidx(s, i, j, k)];
+          *r -= F[fl];
+          *r += F[fr];
+        }
+  }
+}
+
+static void euler_prim_to_ent_all(tor_solver* s) {
+  double G = s->c.gamma;
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < s->npad; ++p) prim_to_ent(G, &s->P[p], &s->V[p]);
+}
+
+static void euler_residual(tor_solver* s) {
+  int g = s->ng;
+  int64_t N = s->ncell;
+  for (int64_t q = 0; q < 5 * N; ++q) s->res[q] = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    if (!s->active[d]) continue;
+    int fn[3] = {s->n[0], s->n[1], s->n[2]};
+    fn[d] += 1;
+    int64_t nf = (int64_t)fn[0] * fn[1] * fn[2];
+    double* F = s->Fface;
+#pragma omp parallel for schedule(static)
+    for (int64_t f = 0; f < nf; ++f) {
+      int i = (int)(f % fn[0]);
+      int j = (int)((f / fn[0]) % fn[1]);
+      int k = (int)(f / ((int64_t)fn[0] * fn[1]));
+      prim_t pp[6];
+      ent_t vv[6];
+      for (int l = 0; l < 2 * g; ++l) {
+        int c[3] = {i, j, k};
+        c[d] += l - g;
+        int64_t p = pidx(s, c[0], c[1], c[2]);
+        pp[l] = s->P[p];
+        vv[l] = s->V[p];
+      }
+      euler_face_flux(&s->c, &s->net, pp, vv, g, d, &s->g, F + 5 * f);
+    }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->n[2]; ++k)
+      for (int j = 0; j < s->n[1]; ++j)
+        for (int i = 0; i < s->n[0]; ++i) {
+          int c[3] = {i, j, k};
+          int64_t fl = ((int64_t)c[2] * fn[1] + c[1]) * fn[0] + c[0];
+          c[d] += 1;
+          int64_t fr = ((int64_t)c[2] * fn[1] + c[1]) * fn[0] + c[0];
+          double* r = &s->res[5 * iidx(s, i, j, k)];
+          for (int v = 0; v < 5; ++v) {
+            r[v] -= F[5 * fl + v];
+            r[v] += F[5 * fr + v];
+          }
+        }
+  }
+}
+
+static const double ARK[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0}; /* ext_constants.h:7 */
+static const double BRK[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0}; /* ext_constants.h:8 */
+
+int tor_step(tor_solver* s, double t_stamp, double* dt_out) {
+  double dt = compute_dt(s);
+  if (s->time + dt > t_stamp) dt = t_stamp - s->time; /* fv.cc:1266 */
+  double factor = dt / (s->g.h[0] * s->g.h[1] * s->g.h[2]);
+  int nrk = (s->c.time_scheme == TECNO_TS_RK1) ? 1 : 3;
+  int64_t N = s->ncell;
+  if (s->nvar == 1) {
+    memcpy(s->uold, s->u, s->npad * sizeof(double));
+    for (int rk = 0; rk < nrk; ++rk) {
+      scalar_residual(s);
+      /* update_solution: enflo_scalar fv.cc:463-476 */
+      for (int k = 0; k < s->n[2]; ++k)
+        for (int j = 0; j < s->n[1]; ++j)
+          for (int i = 0; i < s->n[0]; ++i) {
+            int64_t p = pidx(s, i, j, k);
+            s->u[p] *= BRK[rk];
+            s->u[p] += (s->uold[p] * ARK[rk] - factor * BRK[rk] * s->res[iidx(s, i, j, k)]);
+          }
+      scalar_bc(s, s->u);
+    }
+  } else {
+    memcpy(s->Uo, s->U, 5 * N * sizeof(double));
+    double G = s->c.gamma;
+    for (int rk = 0; rk < nrk; ++rk) {
+      euler_prim_to_ent_all(s);
+      euler_residual(s);
+      /* update_solution: enflo fv.cc:1163-1175 (ConVar::sadd, convar.h:118) */
+      double c2 = -factor * BRK[rk];
+      for (int64_t q = 0; q < N; ++q)
+        for (int v = 0; v < 5; ++v) {
+          s->U[5 * q + v] *= BRK[rk];
+          s->U[5 * q + v] += s->Uo[5 * q + v] * ARK[rk] + s->res[5 * q + v] * c2;
+        }
+      /* con_to_prim: fv.cc:388-400 */
+      for (int k = 0; k < s->n[2]; ++k)
+        for (int j = 0; j < s->n[1]; ++j)
+          for (int i = 0; i < s->n[0]; ++i) {
+            const double* U = s->U + 5 * iidx(s, i, j, k);
+            prim_t* P = &s->P[pidx(s, i, j, k)];
+            P->rho = U[0];
+            double inv = 1.0 / U[0];
+            for (int d = 0; d < 3; ++d) P->u[d] = U[1 + d] * inv;
+            double m2 = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+            P->p = (G - 1.0) * (U[4] - 0.5 * m2 / U[0]);
+          }
+      euler_bc(s);
+    }
+  }
+  s->time += dt;
+  s->iter++;
+  if (dt_out) *dt_out = dt;
+  return 0;
+}
+
+int tor_advance(tor_solver* s, double final_time, int t_stamps, int max_iter,
+                int* iters_out, double* time_out) {
+  double t_int = final_time / t_stamps;
+  double t_save = t_int;
+  /* stamps already passed (when called on a running solver) */
+  while (s->time >= t_save && t_save < final_time) t_save += t_int;
+  while (s->time < final_time && s->iter < max_iter) {
+    double dt;
+    tor_step(s, t_save, &dt);
+    if (s->time == t_save) t_save += t_int;
+  }
+  if (iters_out) *iters_out = s->iter;
+  if (time_out) *time_out = s->time;
+  return 0;
+}
+
+void tor_audit(const tor_solver* s, double out[8]) {
+  double dV = s->g.h[0] * s->g.h[1] * s->g.h[2];
+  for (int v = 0; v < 8; ++v) out[v] = 0.0;
+  double ent = 0.0, ke = 0.0, m = 0.0, mo[3] = {0, 0, 0}, e = 0.0, smax = 0.0;
+  for (int i = 0; i < s->n[0]; ++i)
+    for (int j = 0; j < s->n[1]; ++j)
+      for (int k = 0; k < s->n[2]; ++k) {
+        int64_t p = pidx(s, i, j, k);
+        if (s->nvar == 1) {
+          double u = s->u[p];
+          ent += 0.5 * pow(u, 2);
+          m += u;
+          double sp = (s->c.model == TECNO_MODEL_LINADV)
+                          ? sqrt(s->c.vel[0] * s->c.vel[0] + s->c.vel[1] * s->c.vel[1] +
+                                 s->c.vel[2] * s->c.vel[2])
+                          : fabs(u);
+          smax = dmax(smax, sp);
+        } else {
+          const prim_t* P = &s->P[p];
+          double G = s->c.gamma;
+          double sv = log(P->p) - G * log(P->rho);
+          ent += -P->rho * sv / (G - 1.0);
+          ke += P->rho * vsq(P->u);
+          const double* U = s->U + 5 * iidx(s, i, j, k);
+          m += U[0];
+          for (int d = 0; d < 3; ++d) mo[d] += U[1 + d];
+          e += U[4];
+          smax = dmax(smax, sqrt(vsq(P->u)) + sqrt(G * P->p / P->rho));
+        }
+      }
+  out[0] = ent * dV;
+  out[1] = 0.5 * ke * dV;
+  out[2] = m * dV;
+  out[3] = mo[0] * dV;
+  out[4] = mo[1] * dV;
+  out[5] = mo[2] * dV;
+  out[6] = e * dV;
+  out[7] = smax;
+}

@@ -1,0 +1,107 @@
+"""Configurations of the golden cases (tests/golden/make_fixtures.py writes
+the matching param.in files; these are the same fields as tecno_cfg)."""
+import math
+import os
+
+import numpy as np
+
+from tecno import make_cfg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+_cache = {}
+
+
+def fixtures(name):
+    if name not in _cache:
+        _cache[name] = np.load(os.path.join(GOLDEN, f"fixtures_{name}.npz"))
+    return _cache[name]
+
+
+def case(fx, key):
+    d = fixtures(fx)
+    return {k.split("/", 1)[1]: d[k] for k in d.files if k.split("/", 1)[0] == key}
+
+
+SC1D = dict(system="scalar", lo=(-math.pi, 0.0, 0.0), hi=(math.pi, 1.0, 1.0), dims=1, cfl=0.5)
+FINAL_1D = 0.5
+
+
+def scalar_cfg(key, **kw):
+    """tecno_cfg of the scalar golden case `key` (fixtures_scalar.npz)."""
+    recs = {"spweno": "sp_weno", "spwenoc": "sp_wenoc", "spwenocorr2": "sp_weno_corr2",
+            "eno2": "eno 2", "eno3": "eno 3", "dsp": "sp_weno_dl"}
+    if key.startswith("sc1d_") and key[5:] in recs:
+        c = make_cfg(n=(64, 2, 2), recon=recs[key[5:]], flux="lin_tecno4", **SC1D, **kw)
+        return c, 0.5, 5
+    if key == "config1_spweno":
+        return make_cfg(n=(400, 2, 2), recon="sp_weno", flux="lin_tecno4", **SC1D, **kw), 0.5, 5
+    if key.startswith("sc1d_"):
+        flux = key[5:]
+        return make_cfg(n=(64, 2, 2), recon="sp_weno", flux=flux, **SC1D, **kw), 0.5, 5
+    if key.startswith("sc2d_"):
+        tag = key[5:]
+        base = dict(system="scalar", lo=(-1.0, -1.0, 0.0), hi=(1.0, 1.0, 1.0), n=(24, 20, 2), dims=2,
+                    recon="sp_weno", cfl=0.5)
+        if tag == "bur2d":
+            return make_cfg(flux="bur_tecno4", **base, **kw), 0.3, 5
+        if tag == "lin2d":
+            return make_cfg(flux="lin_tecno4", vel=(1.0, 0.5, 0.0), **base, **kw), 0.3, 5
+        if tag == "lin2d_wall":
+            return make_cfg(flux="lin_tecno4", vel=(1.0, -0.7, 0.0),
+                            bc=["wall", "neumann", "neumann", "wall", "periodic", "periodic"],
+                            **base, **kw), 0.3, 5
+    if key == "sc3d_lin":
+        return make_cfg(system="scalar", lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(10, 8, 6), dims=3,
+                        recon="sp_weno", flux="lin_tecno4", vel=(1.0, 0.5, -0.25), cfl=0.5, **kw), 0.2, 5
+    raise KeyError(key)
+
+
+def euler_cfg(key, **kw):
+    """tecno_cfg of the Euler golden case `key` (fixtures_euler.npz)."""
+    recs = {"spweno": "sp_weno", "spwenoc": "sp_wenoc", "eno3": "eno 3", "dsp": "sp_weno_dl"}
+    if key.startswith("eu2d_riemann_"):
+        return make_cfg(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(20, 16, 2), dims=2,
+                        recon=recs[key[13:]], flux="kepes_tecno4_roe", bc="neumann", **kw), 0.25, 5
+    if key.startswith("eu2d_kh_"):
+        return make_cfg(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(24, 24, 2), dims=2,
+                        recon=recs[key[8:]], flux="kepes_tecno4_roe", bc="periodic", **kw), 1.0, 5
+    if key.startswith("eu2d_vortex_"):
+        tag = key[12:]
+        ec1 = tag.endswith("_ec1")
+        flux = tag[:-4] if ec1 else tag
+        return make_cfg(lo=(-5.0, -5.0, 0.0), hi=(5.0, 5.0, 1.0), n=(20, 20, 2), dims=2, recon="sp_weno",
+                        flux=flux, bc="periodic", ec1_fix=int(ec1), **kw), 10.0, 5
+    if key == "eu3d_tg":
+        return make_cfg(lo=(0.0, 0.0, 0.0), hi=(2 * math.pi,) * 3, n=(8, 6, 6), dims=3, recon="sp_weno",
+                        flux="kepes_tecno4_roe", bc="periodic", **kw), 1.0, 5
+    if key == "eu2d_wall":
+        return make_cfg(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(16, 12, 2), dims=2, recon="sp_weno",
+                        flux="kepes_tecno4_roe",
+                        bc=["wall", "wall", "neumann", "wall", "periodic", "periodic"], **kw), 0.25, 5
+    raise KeyError(key)
+
+
+def collapse(state, dims):
+    """Fixture state (nv, nz, ny, nx) of a run with duplicated dummy layers ->
+    the layers of a `dims`-dimensional grid (takes layer 0 of collapsed axes
+    after checking the duplicates agree)."""
+    s = np.asarray(state)
+    if dims <= 2:
+        assert np.array_equal(s[:, 0], s[:, -1])
+        s = s[:, :1]
+    if dims <= 1:
+        assert np.array_equal(s[:, :, 0], s[:, :, -1])
+        s = s[:, :, :1]
+    return s
+
+
+def euler_prim_for(state, dims):
+    """(5, nz, ny, nx) primitive -> the variables a `dims`-D context holds
+    (2D drops w: rho, u, v, p)."""
+    s = collapse(state, dims)
+    if dims == 2:
+        assert np.all(s[3] == 0.0)
+        s = s[[0, 1, 2, 4]]
+    return s.reshape(s.shape[0], -1)

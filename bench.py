@@ -1,0 +1,283 @@
+#!/usr/bin/env python3
+"""Headline benchmark: 2D Euler Kelvin-Helmholtz shear layer on 2048^2
+(periodic [0,1]^2, rho = 2 / u = -0.5 for |y-0.5| < 0.25, rho = 1 / u = 0.5
+elsewhere, p = 2.5, v = 0.01 sum_{k=1..8} a_k sin(2 pi k x), a_k ~ N(0,1)
+seeded), kepes_tecno4_roe TeCNO flux, SSPRK3 at CFL 0.5.
+
+A step is one full SSPRK3 step of the whole grid (CFL dt + three fused stage
+kernels) with the state resident in HBM.  The headline path is DSP-WENO
+reconstruction (sp_weno_dl, the shipped 5-5-5-5-5 network) with FP32 flux and
+reconstruction; the FP64 SP-WENO path is timed beside it.  Prints one JSON
+line (metric, value in cell-updates/s, roofline of the stage kernel, CPU
+baseline).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 2048]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dsp-weno_amd"))
+
+METRIC = "cell-updates/s, 2D Euler 2048² DSP-WENO mixed vs FP64 SP-WENO at equal L1 err"
+NET_FILE = os.path.join(ROOT, "tests", "golden", "dsp_weno_net.txt")
+HBM_PEAK_GBS = 8000.0
+NVAR = 4
+
+
+def kh_state(n, seed=2024):
+    a = np.random.default_rng(seed).normal(size=8)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    inner = np.abs(Y - 0.5) < 0.25
+    rho = np.where(inner, 2.0, 1.0)
+    u = np.where(inner, -0.5, 0.5)
+    v = 0.01 * sum(a[k] * np.sin(2 * np.pi * (k + 1) * X) for k in range(8))
+    p = np.full_like(rho, 2.5)
+    return np.stack([rho, u, v, p]).reshape(4, -1), a
+
+
+def run_variant(tecno, n, steps, warmup, precision, recon, mlp_dtype):
+    cfg = tecno.make_cfg(n=(n, n, 2), dims=2, lo=(0, 0, 0), hi=(1, 1, 1), recon=recon,
+                         flux="kepes_tecno4_roe", bc="periodic", cfl=0.5, precision=precision,
+                         mlp_dtype=mlp_dtype)
+    s = tecno.Solver(cfg)
+    if recon == "sp_weno_dl":
+        s.load_net(NET_FILE)
+    prim, _ = kh_state(n)
+    s.set_state(prim)
+    s.run_steps(warmup, final_time=1.0, t_stamps=1)
+    s.sync()
+    s.timing(True)
+    t0 = time.perf_counter()
+    s.run_steps(steps, final_time=1.0, t_stamps=1)
+    s.sync()
+    el = time.perf_counter() - t0
+    stage_ms, nst = s.timing_get()
+    s.timing(False)
+    a = s.audit()
+    t, it = s.time()
+    s.close()
+    if a.n_nonfinite:
+        raise RuntimeError(f"non-finite state in {recon}/{precision}")
+    return dict(seconds=el, cell_updates_per_s=n * n * steps / el, stage_ms=stage_ms,
+                n_stages=nst, ms_per_step=1e3 * el / steps, time=t, iters=it)
+
+
+KH_PARAM = """grid
+{{
+   x_min   0.0
+   x_max   1.0
+   Nx_cell {{ {n} }}
+   y_min   0.0
+   y_max   1.0
+   Ny_cell {{ {n} }}
+   z_min   0.0
+   z_max   1.0
+   Nz_cell {{ 2 }}
+}}
+numeric
+{{
+   time_scheme  ssprk3
+   cfl          0.5
+   max_iter     50000
+   final_time   1.0
+   min_residue  1.0e-6
+   reconstruct  {rec}
+   net_file     {net}
+}}
+material
+{{
+   gamma       1.4
+   gas_const   1.0
+   viscosity
+   {{
+      model    constant
+      mu_ref   0.0
+   }}
+   prandtl     0.72
+   model       euler
+   flux        kepes_tecno4_roe
+   EC1_fix     no
+}}
+constants
+{{
+{consts}
+}}
+initial_condition
+{{
+   density    2.0*(abs(y-0.5)<0.25) + 1.0*(abs(y-0.5)>=0.25)
+   xvelocity  -0.5*(abs(y-0.5)<0.25) + 0.5*(abs(y-0.5)>=0.25)
+   yvelocity  0.01*({pert})
+   zvelocity  0.0
+   pressure   2.5
+}}
+exact_soln
+{{
+   available   no
+}}
+boundary_condition
+{{
+   xleft
+   {{
+      type periodic
+   }}
+   xright
+   {{
+      type periodic
+   }}
+   yleft
+   {{
+      type periodic
+   }}
+   yright
+   {{
+      type periodic
+   }}
+   zleft
+   {{
+      type periodic
+   }}
+   zright
+   {{
+      type periodic
+   }}
+}}
+output
+{{
+   OOC_study   no
+   problem_dim 2
+   time_stamps 1
+   output_path {out}
+}}
+"""
+
+
+def cpu_baseline(n_sample=512, steps=2):
+    """The reference solver (built from its sources into oracle/_ref by
+    oracle/Makefile) on one host core, on a bounded sample of the workload:
+    the KH problem at n_sample^2 (the reference's Nz = 2 layout), `steps`
+    steps of sp_weno_dl.  Falls back to the oracle's C port when the
+    reference build is absent."""
+    refh = os.path.join(ROOT, "oracle", "_ref", "refh_sys")
+    a = np.random.default_rng(2024).normal(size=8)
+    if os.path.exists(refh):
+        with tempfile.TemporaryDirectory() as tmp:
+            consts = "\n".join(f"   a{k}   {float(a[k])!r}" for k in range(8))
+            pert = " + ".join(f"a{k}*sin(2*pi*{k + 1}*x)" for k in range(8))
+            p = os.path.join(tmp, "param.in")
+            with open(p, "w") as f:
+                f.write(KH_PARAM.format(n=n_sample, rec="sp_weno_dl", net=NET_FILE, consts=consts,
+                                        pert=pert, out=tmp))
+            r = subprocess.run(["taskset", "-c", "0", refh, "time", p, str(steps)], capture_output=True,
+                               text=True, timeout=600)
+            if r.returncode == 0:
+                tok = r.stdout.split()
+                sec = float(tok[3])
+                return {"value": n_sample * n_sample * steps / sec, "unit": "cell-updates/s",
+                        "cores": 1, "kind": "reference",
+                        "sample": f"KH {n_sample}x{n_sample} (x2 z-layers as the reference stores 2D), "
+                                  f"{steps} SSPRK3 step(s), sp_weno_dl FP64, reference solver built from "
+                                  f"source, 1 host core, {sec:.1f} s"}
+    # fallback: the oracle (C restatement), faithful mode, one thread
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as orc
+    import tecno
+    cfg = tecno.make_cfg(n=(n_sample, n_sample, 2), dims=2, recon="sp_weno_dl", bc="periodic")
+    s = orc.OracleSolver(cfg, orc.load_net(NET_FILE), faithful=True, threads=1)
+    prim, _ = kh_state(n_sample)
+    full = np.zeros((5, 2, n_sample * n_sample))
+    full[[0, 1, 2, 4]] = prim[:, None, :]
+    s.set_state(full.reshape(5, -1))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(1.0)
+    sec = time.perf_counter() - t0
+    return {"value": n_sample * n_sample * steps / sec, "unit": "cell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"KH {n_sample}x{n_sample}x2, {steps} step(s), sp_weno_dl, oracle C port, 1 core"}
+
+
+def load_traffic(variant):
+    p = os.path.join(ROOT, "profiles", f"traffic_{variant}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="headline | fp64 (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import tecno
+
+    n = args.n
+    head_prec, head_rec, head_mlp = tecno.FP32, "sp_weno_dl", tecno.MLP_FP32
+    res = {}
+    if args.only in ("", "headline"):
+        res["headline"] = run_variant(tecno, n, args.steps, args.warmup, head_prec, head_rec, head_mlp)
+        print(f"# headline (FP32, sp_weno_dl): {res['headline']}", file=sys.stderr, flush=True)
+    if args.only in ("", "fp64"):
+        res["fp64_spweno"] = run_variant(tecno, n, args.steps, args.warmup, tecno.FP64, "sp_weno",
+                                         tecno.MLP_FP64)
+        print(f"# FP64 sp_weno: {res['fp64_spweno']}", file=sys.stderr, flush=True)
+    if args.only:
+        return
+    h = res["headline"]
+    b = res["fp64_spweno"]
+    # algorithmic HBM bytes per stage launch: 8 state passes of NVAR values per
+    # cell-update (read U^(k-1) [and U^n], write U^(k)), one third per stage
+    esize = 4
+    bytes_per_launch = n * n * 8 * NVAR * esize / 3.0
+    achieved = bytes_per_launch / (h["stage_ms"] * 1e-3) / 1e9
+    traffic = load_traffic("headline")
+    out = {
+        "metric": METRIC,
+        "value": h["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded KH initial state; shipped DSP-WENO weights)",
+        "config": {"workload": f"2D Euler Kelvin-Helmholtz {n}x{n} periodic, kepes_tecno4_roe + "
+                               f"sp_weno_dl (DSP-WENO), SSPRK3 CFL 0.5",
+                   "cells": n * n, "flux": "kepes_tecno4_roe", "reconstruct": "sp_weno_dl",
+                   "precision": "fp32 flux/reconstruction/state, fp32 SIMT MLP"},
+        "fp64_spweno_cell_updates_per_s": b["cell_updates_per_s"],
+        "speedup_vs_fp64_spweno": h["cell_updates_per_s"] / b["cell_updates_per_s"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "kernel": "stage_kernel<Euler2Ph<float,2,KEPES4_ROE,DSP>>",
+                     "stage_ms": h["stage_ms"], "bytes_per_launch_alg": bytes_per_launch},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,839 @@
+// capi.cpp — the C ABI of libtecno.so (include/tecno.h): contexts, device
+// buffers, time stepping, net_file handling, batched seams, timing.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/tecno.h"
+#include "fv_launch.hpp"
+#include "mlp.hpp"
+
+namespace tecno {
+extern template struct Launch<double>;
+extern template struct Launch<float>;
+}  // namespace tecno
+
+using namespace tecno;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(TECNO_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+struct HostNet {
+  int nl = 0;
+  int w[TECNO_NET_MAXL + 1] = {0};
+  std::vector<double> W = std::vector<double>(TECNO_NET_MAXL * TECNO_NET_MAXW * TECNO_NET_MAXW, 0.0);
+  std::vector<double> b = std::vector<double>(TECNO_NET_MAXL * TECNO_NET_MAXW, 0.0);
+};
+
+struct tecno_ctx {
+  tecno_cfg cfg;
+  int dims = 2;
+  int nvar = 4;    // conserved variables held
+  int nprim = 4;   // primitive variables exchanged with the host
+  int n[3] = {1, 1, 1};
+  int64_t N = 0;
+  int G = 2;
+  double h[3] = {0, 0, 0};
+  double hmin = 0, vol = 0;
+  int esize = 8;   // bytes per state value
+  hipStream_t stream = nullptr;
+  void* U[3] = {nullptr, nullptr, nullptr};
+  Ctl* ctl = nullptr;
+  double* scratch = nullptr;  // host<->device staging, audit partials
+  size_t scratch_bytes = 0;
+  HostNet net;
+  bool has_net = false;
+  double* dW64 = nullptr;
+  double* db64 = nullptr;
+  float* dW32 = nullptr;
+  float* db32 = nullptr;
+  Phys<double> P64;
+  Phys<float> P32;
+  LaunchInfo L;
+  bool state_set = false;
+  bool configured_final = false;
+  double cfg_final = 0;
+  int cfg_stamps = 0;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  double t_ms_acc = 0;
+  int64_t t_n = 0;
+  MlpTrainer* trainer = nullptr;
+};
+
+// ------------------------------------------------------------------ helpers
+static int flux_ng(int flux) {
+  switch (flux) {
+    case TECNO_FLUX_LIN_EC4: case TECNO_FLUX_LIN_TECNO4: case TECNO_FLUX_BUR_EC4:
+    case TECNO_FLUX_BUR_TECNO4: case TECNO_FLUX_KEPES_TECNO4_ROE:
+    case TECNO_FLUX_KEPES_TECNO4_RUSANOV: case TECNO_FLUX_ROE_TECNO4_ROE:
+    case TECNO_FLUX_KEPEC4: case TECNO_FLUX_ROE_EC4:
+      return 2;
+    default:
+      return 1;
+  }
+}
+
+template <typename T>
+static void fill_phys(const tecno_ctx* c, Phys<T>& P) {
+  const tecno_cfg& g = c->cfg;
+  P.system = g.system;
+  P.model = g.model;
+  P.flux = g.flux;
+  P.recon = g.recon == TECNO_REC_SP_WENO ? REC_SPWENO
+            : g.recon == TECNO_REC_ENO   ? REC_ENO
+            : g.recon == TECNO_REC_SP_WENO_DL ? REC_DSP
+                                               : REC_FIRST;
+  P.corr = g.spweno_corr;
+  P.eno_k = g.eno_level;
+  P.ec1_fix = g.ec1_fix;
+  P.fix_zmom = g.fix_zmom;
+  P.sign_fallback = g.sign_fallback;
+  P.mlp_mode = g.mlp_dtype;
+  P.gamma = T(g.gamma);
+  P.R = T(g.gas_const);
+  P.gm1 = T(g.gamma - 1.0);
+  P.inv_gm1 = T(1.0 / (g.gamma - 1.0));
+  for (int d = 0; d < 3; ++d) {
+    P.vel[d] = T(g.vel[d]);
+    P.h[d] = T(c->h[d]);
+  }
+  P.area[0] = T(c->h[1] * c->h[2]);
+  P.area[1] = T(c->h[0] * c->h[2]);
+  P.area[2] = T(c->h[0] * c->h[1]);
+  P.hmin = T(c->hmin);
+  P.cfl = T(g.cfl);
+  P.vol = T(c->vol);
+  P.vol_d = c->vol;
+  P.hmin_d = c->hmin;
+  P.cfl_d = g.cfl;
+  P.s05 = T(std::sqrt(0.5));
+  P.sgm1 = T(std::sqrt(g.gamma - 1.0));
+  P.gpgm = T((g.gamma + 1.0) / (g.gamma - 1.0));
+  P.vel_norm = std::sqrt(g.vel[0] * g.vel[0] + g.vel[1] * g.vel[1] + g.vel[2] * g.vel[2]);
+  P.arch5 = 0;
+  P.net.nl = c->net.nl;
+  for (int i = 0; i <= TECNO_NET_MAXL; ++i) P.net.w[i] = c->net.w[i];
+  if (c->has_net && c->net.nl == 4) {
+    bool all5 = true;
+    for (int i = 0; i <= 4; ++i) all5 = all5 && c->net.w[i] == 5;
+    P.arch5 = all5;
+  }
+}
+
+static void refresh_phys(tecno_ctx* c) {
+  fill_phys(c, c->P64);
+  fill_phys(c, c->P32);
+  c->P64.net.W = c->dW64;
+  c->P64.net.b = c->db64;
+  c->P32.net.W = c->dW32;
+  c->P32.net.b = c->db32;
+}
+
+static int ensure_scratch(tecno_ctx* c, size_t bytes) {
+  if (bytes <= c->scratch_bytes) return 0;
+  if (c->scratch) (void)hipFree(c->scratch);
+  c->scratch = nullptr;
+  c->scratch_bytes = 0;
+  HIPCHK(hipMalloc(&c->scratch, bytes));
+  c->scratch_bytes = bytes;
+  return 0;
+}
+
+static int upload_net(tecno_ctx* c) {
+  const size_t nw = TECNO_NET_MAXL * TECNO_NET_MAXW * TECNO_NET_MAXW, nb = TECNO_NET_MAXL * TECNO_NET_MAXW;
+  if (!c->dW64) {
+    HIPCHK(hipMalloc(&c->dW64, nw * sizeof(double)));
+    HIPCHK(hipMalloc(&c->db64, nb * sizeof(double)));
+    HIPCHK(hipMalloc(&c->dW32, nw * sizeof(float)));
+    HIPCHK(hipMalloc(&c->db32, nb * sizeof(float)));
+  }
+  std::vector<float> w32(c->net.W.begin(), c->net.W.end()), b32(c->net.b.begin(), c->net.b.end());
+  HIPCHK(hipMemcpy(c->dW64, c->net.W.data(), nw * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->db64, c->net.b.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dW32, w32.data(), nw * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->db32, b32.data(), nb * sizeof(float), hipMemcpyHostToDevice));
+  refresh_phys(c);
+  return 0;
+}
+
+static bool fp64(const tecno_ctx* c) { return c->cfg.precision == TECNO_FP64; }
+
+static hipError_t launch_stage(tecno_ctx* c, const void* U, const void* U0, void* Uo, double a,
+                               double b, int last) {
+  if (fp64(c))
+    return Launch<double>::stage(c->L, c->P64, (const double*)U, (const double*)U0, (double*)Uo, a,
+                                 b, last);
+  return Launch<float>::stage(c->L, c->P32, (const float*)U, (const float*)U0, (float*)Uo, a, b,
+                              last);
+}
+
+// ------------------------------------------------------------------ API
+extern "C" {
+
+const char* tecno_last_error(void) { return g_err.c_str(); }
+const char* tecno_version(void) { return "tecno-mi355x 0.1 (gfx950)"; }
+
+void tecno_cfg_default(tecno_cfg* cfg) {
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->system = TECNO_SYS_EULER;
+  cfg->model = TECNO_MODEL_EULER;
+  cfg->flux = TECNO_FLUX_KEPES_TECNO4_ROE;
+  cfg->recon = TECNO_REC_SP_WENO;
+  cfg->eno_level = 2;
+  cfg->time_scheme = TECNO_TS_SSPRK3;
+  cfg->precision = TECNO_FP64;
+  cfg->mlp_dtype = TECNO_MLP_FP64;
+  cfg->dims = 2;
+  for (int d = 0; d < 3; ++d) {
+    cfg->n[d] = 2;
+    cfg->lo[d] = 0.0;
+    cfg->hi[d] = 1.0;
+  }
+  cfg->cfl = 0.5;
+  cfg->vel[0] = 1.0;
+  cfg->gamma = 1.4;
+  cfg->gas_const = 1.0;
+  cfg->prandtl = 0.72;
+}
+
+int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
+  if (!cfg || !out) return fail(TECNO_EINVAL, "null argument");
+  *out = nullptr;
+  const tecno_cfg& g = *cfg;
+  if (g.dims < 1 || g.dims > 3) return fail(TECNO_EINVAL, "dims must be 1, 2 or 3");
+  if (g.system == TECNO_SYS_EULER && g.dims != 2)
+    return fail(TECNO_EINVAL, "Euler: only 2D grids are supported by this build");
+  if (g.system == TECNO_SYS_SCALAR && g.dims == 3)
+    return fail(TECNO_EINVAL, "scalar: 3D grids are not supported by this build");
+  if (g.system == TECNO_SYS_SCALAR && (g.flux < 0 || g.flux > 7))
+    return fail(TECNO_EINVAL, "unknown scalar flux");
+  if (g.system == TECNO_SYS_EULER && !((g.flux >= 100 && g.flux <= 106) || (g.flux >= 120 && g.flux <= 124)))
+    return fail(TECNO_EINVAL, "unknown Euler flux");
+  if (g.system == TECNO_SYS_EULER && g.model != TECNO_MODEL_EULER)
+    return fail(TECNO_EINVAL, "Navier-Stokes models are not supported by this build");
+  if (g.recon == TECNO_REC_ENO && g.eno_level != 2 && g.eno_level != 3)
+    return fail(TECNO_EINVAL, "eno k: k must be 2 or 3");
+  if (g.time_scheme != TECNO_TS_SSPRK3 && g.time_scheme != TECNO_TS_RK1)
+    return fail(TECNO_EINVAL, "only ssprk3 and rk1 time schemes are supported");
+  if (g.precision != TECNO_FP64 && g.precision != TECNO_FP32)
+    return fail(TECNO_EINVAL, "precision must be FP64 or FP32");
+  for (int i = 0; i < 6; ++i)
+    if (g.bc[i] == TECNO_BC_DIRICHLET || g.bc[i] < 0 || g.bc[i] > 3)
+      return fail(TECNO_EINVAL, "boundary type must be periodic, neumann or wall");
+  for (int d = 0; d < 3; ++d) {
+    if (g.n[d] < 1) return fail(TECNO_EINVAL, "cell counts must be positive");
+    if (!(g.hi[d] > g.lo[d])) return fail(TECNO_EINVAL, "empty domain");
+  }
+  if (!(g.cfl > 0)) return fail(TECNO_EINVAL, "cfl must be positive");
+
+  tecno_ctx* c = new tecno_ctx();
+  c->cfg = g;
+  c->dims = g.dims;
+  for (int d = 0; d < 3; ++d) {
+    c->h[d] = (g.hi[d] - g.lo[d]) / g.n[d];
+    c->n[d] = d < g.dims ? g.n[d] : 1;
+  }
+  c->hmin = std::min(c->h[0], std::min(c->h[1], c->h[2]));
+  c->vol = c->h[0] * c->h[1] * c->h[2];
+  c->N = (int64_t)c->n[0] * c->n[1] * c->n[2];
+  c->nvar = g.system == TECNO_SYS_SCALAR ? 1 : (g.dims == 3 ? 5 : 4);
+  c->nprim = c->nvar;
+  int rec_ng = g.recon == TECNO_REC_ENO ? g.eno_level : 2;
+  c->G = std::max(flux_ng(g.flux), rec_ng);
+  if (c->G < 2) c->G = 2;
+  for (int d = 0; d < g.dims; ++d)
+    if (c->n[d] < c->G) {
+      delete c;
+      return fail(TECNO_EINVAL, "grid too small for the stencil");
+    }
+  c->esize = fp64(c) ? 8 : 4;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(TECNO_EHIP, "hipStreamCreate failed");
+  }
+  size_t bytes = (size_t)c->N * c->nvar * c->esize;
+  for (int i = 0; i < 3; ++i)
+    if (hipMalloc(&c->U[i], bytes) != hipSuccess) {
+      tecno_destroy(c);
+      return fail(TECNO_ENOMEM, "device allocation of the state failed");
+    }
+  if (hipMalloc(&c->ctl, sizeof(Ctl)) != hipSuccess) {
+    tecno_destroy(c);
+    return fail(TECNO_ENOMEM, "device allocation failed");
+  }
+  Ctl h{};
+  (void)hipMemcpy(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice);
+  c->L.system = g.system == TECNO_SYS_SCALAR ? 0 : 1;
+  c->L.dims = g.dims;
+  c->L.G = c->G;
+  c->L.fast_flux = g.flux == TECNO_FLUX_KEPES_TECNO4_ROE && !g.ec1_fix ? FV_KEPES4_ROE : FV_GENERIC;
+  c->L.fast_rec = (g.recon == TECNO_REC_SP_WENO) ? RV_SPWENO
+                  : (g.recon == TECNO_REC_SP_WENO_DL) ? RV_DSP
+                                                      : RV_GENERIC;
+  for (int d = 0; d < 3; ++d) c->L.grid.n[d] = c->n[d];
+  for (int i = 0; i < 6; ++i) c->L.grid.bc[i] = g.bc[i];
+  c->L.grid.N = c->N;
+  c->L.stream = c->stream;
+  c->L.ctl = c->ctl;
+  refresh_phys(c);
+  *out = c;
+  return 0;
+}
+
+void tecno_destroy(tecno_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 3; ++i)
+    if (c->U[i]) (void)hipFree(c->U[i]);
+  if (c->ctl) (void)hipFree(c->ctl);
+  if (c->scratch) (void)hipFree(c->scratch);
+  if (c->dW64) (void)hipFree(c->dW64);
+  if (c->db64) (void)hipFree(c->db64);
+  if (c->dW32) (void)hipFree(c->dW32);
+  if (c->db32) (void)hipFree(c->db32);
+  for (auto& e : c->ev) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  delete c->trainer;
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// net_file reader: Neural_Network::load_network (enflo/src/neural_network.cc:82-130)
+static int parse_net(const char* path, HostNet& net, std::string& err) {
+  std::ifstream in(path);
+  if (!in.good()) {
+    err = std::string("DSP-WENO network file does not exist: ") + path;
+    return TECNO_EIO;
+  }
+  net = HostNet();
+  bool weight = true;
+  int nw = 0, nb = 0;
+  std::string line;
+  while (std::getline(in, line)) {
+    while (!line.empty() && (line.back() == '\r' || line.back() == ' ')) line.pop_back();
+    if (line == "BEGIN") {
+      std::vector<std::vector<double>> rows;
+      std::vector<double> bias;
+      while (std::getline(in, line)) {
+        while (!line.empty() && (line.back() == '\r' || line.back() == ' ')) line.pop_back();
+        if (line == "END") break;
+        std::istringstream iss(line);
+        std::vector<double> row;
+        double v;
+        while (iss >> v) row.push_back(v);
+        if (weight)
+          rows.push_back(row);
+        else
+          bias.insert(bias.end(), row.begin(), row.end());
+      }
+      if (weight) {
+        if (nw >= TECNO_NET_MAXL || rows.empty()) { err = "net_file: bad weight block"; return TECNO_EIO; }
+        int out = (int)rows.size(), inn = (int)rows[0].size();
+        if (out > TECNO_NET_MAXW || inn > TECNO_NET_MAXW) { err = "net_file: layer wider than 16"; return TECNO_EIO; }
+        for (int o = 0; o < out; ++o) {
+          if ((int)rows[o].size() != inn) { err = "net_file: ragged weight block"; return TECNO_EIO; }
+          for (int i = 0; i < inn; ++i) net.W[(size_t)nw * TECNO_NET_MAXW * TECNO_NET_MAXW + o * TECNO_NET_MAXW + i] = rows[o][i];
+        }
+        if (nw > 0 && net.w[nw] != inn) { err = "net_file: layer widths do not chain"; return TECNO_EIO; }
+        net.w[nw] = inn;
+        net.w[nw + 1] = out;
+        nw++;
+      } else {
+        if (nb >= nw || (int)bias.size() != net.w[nb + 1]) { err = "net_file: bias block does not match its layer"; return TECNO_EIO; }
+        for (size_t o = 0; o < bias.size(); ++o) net.b[(size_t)nb * TECNO_NET_MAXW + o] = bias[o];
+        nb++;
+      }
+    } else if (line == "Biases") {
+      weight = false;
+    }
+  }
+  if (nw == 0 || nb != nw) { err = "net_file: weight and bias blocks do not match"; return TECNO_EIO; }
+  if (net.w[0] != 5 || net.w[nw] != 5) { err = "net_file: the network must map 5 features to 5 vertex weights"; return TECNO_EIO; }
+  net.nl = nw;
+  return 0;
+}
+
+int tecno_load_net(tecno_ctx* c, const char* path) {
+  if (!c || !path) return fail(TECNO_EINVAL, "null argument");
+  HostNet net;
+  std::string err;
+  int rc = parse_net(path, net, err);
+  if (rc) return fail(rc, err);
+  c->net = net;
+  c->has_net = true;
+  return upload_net(c);
+}
+
+int tecno_set_net(tecno_ctx* c, int nl, const int32_t* widths, const double* W, const double* b) {
+  if (!c || !widths || !W || !b) return fail(TECNO_EINVAL, "null argument");
+  if (nl < 1 || nl > TECNO_NET_MAXL) return fail(TECNO_EINVAL, "bad layer count");
+  HostNet net;
+  net.nl = nl;
+  size_t ow = 0, ob = 0;
+  for (int l = 0; l <= nl; ++l) {
+    if (widths[l] < 1 || widths[l] > TECNO_NET_MAXW) return fail(TECNO_EINVAL, "bad layer width");
+    net.w[l] = widths[l];
+  }
+  if (net.w[0] != 5 || net.w[nl] != 5) return fail(TECNO_EINVAL, "network must map 5 -> 5");
+  for (int l = 0; l < nl; ++l) {
+    for (int o = 0; o < net.w[l + 1]; ++o) {
+      for (int i = 0; i < net.w[l]; ++i)
+        net.W[(size_t)l * TECNO_NET_MAXW * TECNO_NET_MAXW + o * TECNO_NET_MAXW + i] = W[ow + o * net.w[l] + i];
+      net.b[(size_t)l * TECNO_NET_MAXW + o] = b[ob + o];
+    }
+    ow += (size_t)net.w[l] * net.w[l + 1];
+    ob += net.w[l + 1];
+  }
+  c->net = net;
+  c->has_net = true;
+  return upload_net(c);
+}
+
+int tecno_get_net(const tecno_ctx* c, int* nl, int32_t* widths, double* W, double* b) {
+  if (!c || !nl || !widths || !W || !b) return fail(TECNO_EINVAL, "null argument");
+  if (!c->has_net) return fail(TECNO_ESTATE, "no network loaded");
+  *nl = c->net.nl;
+  size_t ow = 0, ob = 0;
+  for (int l = 0; l <= c->net.nl; ++l) widths[l] = c->net.w[l];
+  for (int l = 0; l < c->net.nl; ++l) {
+    for (int o = 0; o < c->net.w[l + 1]; ++o) {
+      for (int i = 0; i < c->net.w[l]; ++i)
+        W[ow + o * c->net.w[l] + i] = c->net.W[(size_t)l * TECNO_NET_MAXW * TECNO_NET_MAXW + o * TECNO_NET_MAXW + i];
+      b[ob + o] = c->net.b[(size_t)l * TECNO_NET_MAXW + o];
+    }
+    ow += (size_t)c->net.w[l] * c->net.w[l + 1];
+    ob += c->net.w[l + 1];
+  }
+  return 0;
+}
+
+int64_t tecno_num_cells(const tecno_ctx* c) { return c ? c->N : 0; }
+int tecno_num_vars(const tecno_ctx* c) { return c ? c->nprim : 0; }
+
+static int need_net(tecno_ctx* c) {
+  if (c->cfg.recon == TECNO_REC_SP_WENO_DL && !c->has_net)
+    return fail(TECNO_ESTATE, "sp_weno_dl needs a network (tecno_load_net)");
+  return 0;
+}
+
+int tecno_set_state(tecno_ctx* c, const double* prim, int64_t n) {
+  if (!c || !prim) return fail(TECNO_EINVAL, "null argument");
+  if (n != c->N) return fail(TECNO_EINVAL, "cell count mismatch");
+  size_t bytes = (size_t)c->N * c->nprim * sizeof(double);
+  if (int rc = ensure_scratch(c, bytes)) return rc;
+  HIPCHK(hipMemcpyAsync(c->scratch, prim, bytes, hipMemcpyHostToDevice, c->stream));
+  if (fp64(c))
+    HIPCHK(Launch<double>::prim_to_con(c->L, c->scratch, (double*)c->U[0], c->cfg.gamma));
+  else
+    HIPCHK(Launch<float>::prim_to_con(c->L, c->scratch, (float*)c->U[0], c->cfg.gamma));
+  Ctl h{};
+  h.time = 0.0;
+  h.iter = 0;
+  h.final_time = INFINITY;
+  h.max_iter = 0x7fffffff;
+  h.t_save = INFINITY;
+  h.t_int = 0.0;
+  HIPCHK(hipMemcpyAsync(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+  if (!(c->cfg.system == TECNO_SYS_SCALAR && c->cfg.model == TECNO_MODEL_LINADV)) {
+    if (fp64(c))
+      HIPCHK(Launch<double>::speed(c->L, c->P64, (const double*)c->U[0]));
+    else
+      HIPCHK(Launch<float>::speed(c->L, c->P32, (const float*)c->U[0]));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->state_set = true;
+  c->configured_final = false;
+  return 0;
+}
+
+int tecno_get_state(tecno_ctx* c, double* prim, int64_t n) {
+  if (!c || !prim) return fail(TECNO_EINVAL, "null argument");
+  if (n != c->N) return fail(TECNO_EINVAL, "cell count mismatch");
+  if (!c->state_set) return fail(TECNO_ESTATE, "no state");
+  size_t bytes = (size_t)c->N * c->nprim * sizeof(double);
+  if (int rc = ensure_scratch(c, bytes)) return rc;
+  if (fp64(c))
+    HIPCHK(Launch<double>::con_to_prim(c->L, (const double*)c->U[0], c->scratch, c->cfg.gamma));
+  else
+    HIPCHK(Launch<float>::con_to_prim(c->L, (const float*)c->U[0], c->scratch, c->cfg.gamma));
+  HIPCHK(hipMemcpyAsync(prim, c->scratch, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// one step, asynchronous (device-side dt and time)
+static int launch_step(tecno_ctx* c) {
+  const bool linadv = c->cfg.system == TECNO_SYS_SCALAR && c->cfg.model == TECNO_MODEL_LINADV;
+  HIPCHK(launch_dt(c->L, c->hmin, c->cfg.cfl, linadv ? 1 : 0, c->P64.vel_norm));
+  void *A = c->U[0], *B = c->U[1], *C = c->U[2];
+  auto timed = [&](auto fn) -> int {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      HIPCHK(hipEventRecord(e0, c->stream));
+    }
+    HIPCHK(fn());
+    if (c->timing) {
+      HIPCHK(hipEventRecord(e1, c->stream));
+      c->ev.emplace_back(e0, e1);
+    }
+    return 0;
+  };
+  int rc;
+  if (c->cfg.time_scheme == TECNO_TS_RK1) {
+    if ((rc = timed([&] { return launch_stage(c, A, A, B, 0.0, 1.0, 1); }))) return rc;
+    HIPCHK(hipMemcpyAsync(A, B, (size_t)c->N * c->nvar * c->esize, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    // SSPRK3 (ext_constants.h:7-8): a = {0, 3/4, 1/3}, b = {1, 1/4, 2/3}
+    if ((rc = timed([&] { return launch_stage(c, A, A, B, 0.0, 1.0, 0); }))) return rc;
+    if ((rc = timed([&] { return launch_stage(c, B, A, C, 3.0 / 4.0, 1.0 / 4.0, 0); }))) return rc;
+    if ((rc = timed([&] { return launch_stage(c, C, A, A, 1.0 / 3.0, 2.0 / 3.0, 1); }))) return rc;
+  }
+  HIPCHK(launch_finish(c->L));
+  return 0;
+}
+
+static int set_schedule(tecno_ctx* c, double final_time, int t_stamps, int max_iter) {
+  Ctl h{};
+  HIPCHK(hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  h.final_time = final_time;
+  h.t_int = final_time / t_stamps;
+  h.t_save = h.t_int;
+  while (h.time >= h.t_save && h.t_save < final_time) h.t_save += h.t_int;
+  h.max_iter = max_iter;
+  HIPCHK(hipMemcpyAsync(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+  c->configured_final = true;
+  c->cfg_final = final_time;
+  c->cfg_stamps = t_stamps;
+  return 0;
+}
+
+int tecno_step(tecno_ctx* c, double t_stamp, double* dt_out) {
+  if (!c) return fail(TECNO_EINVAL, "null argument");
+  if (!c->state_set) return fail(TECNO_ESTATE, "tecno_set_state first");
+  if (int rc = need_net(c)) return rc;
+  Ctl h{};
+  HIPCHK(hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  h.final_time = INFINITY;
+  h.max_iter = 0x7fffffff;
+  h.t_save = t_stamp;
+  h.t_int = 0.0;
+  HIPCHK(hipMemcpyAsync(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+  c->configured_final = false;
+  if (int rc = launch_step(c)) return rc;
+  HIPCHK(hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (dt_out) *dt_out = h.dt;
+  if (h.nbad) return fail(TECNO_ENONFINITE, "non-finite or non-positive density/pressure");
+  return 0;
+}
+
+int tecno_run_steps(tecno_ctx* c, int n, double final_time, int t_stamps) {
+  if (!c) return fail(TECNO_EINVAL, "null argument");
+  if (!c->state_set) return fail(TECNO_ESTATE, "tecno_set_state first");
+  if (int rc = need_net(c)) return rc;
+  if (t_stamps < 1) return fail(TECNO_EINVAL, "t_stamps must be >= 1");
+  if (!c->configured_final || c->cfg_final != final_time || c->cfg_stamps != t_stamps)
+    if (int rc = set_schedule(c, final_time, t_stamps, 0x7fffffff)) return rc;
+  for (int i = 0; i < n; ++i)
+    if (int rc = launch_step(c)) return rc;
+  return 0;
+}
+
+int tecno_advance(tecno_ctx* c, double final_time, int t_stamps, int max_iter, int* iters_out,
+                  double* time_out) {
+  if (!c) return fail(TECNO_EINVAL, "null argument");
+  if (!c->state_set) return fail(TECNO_ESTATE, "tecno_set_state first");
+  if (int rc = need_net(c)) return rc;
+  if (t_stamps < 1) return fail(TECNO_EINVAL, "t_stamps must be >= 1");
+  if (int rc = set_schedule(c, final_time, t_stamps, max_iter)) return rc;
+  Ctl h{};
+  for (;;) {
+    for (int i = 0; i < 32; ++i)
+      if (int rc = launch_step(c)) return rc;
+    HIPCHK(hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (h.nbad) return fail(TECNO_ENONFINITE, "non-finite or non-positive density/pressure");
+    if (!(h.time < h.final_time && h.iter < h.max_iter)) break;
+  }
+  if (iters_out) *iters_out = h.iter;
+  if (time_out) *time_out = h.time;
+  return 0;
+}
+
+int tecno_get_time(tecno_ctx* c, double* t, int* it) {
+  if (!c) return fail(TECNO_EINVAL, "null argument");
+  Ctl h{};
+  HIPCHK(hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (t) *t = h.time;
+  if (it) *it = h.iter;
+  return 0;
+}
+
+int tecno_synchronize(tecno_ctx* c) {
+  if (!c) return fail(TECNO_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int tecno_audit_state(tecno_ctx* c, tecno_audit* out) {
+  if (!c || !out) return fail(TECNO_EINVAL, "null argument");
+  if (!c->state_set) return fail(TECNO_ESTATE, "no state");
+  const int nblk = 1024;
+  if (int rc = ensure_scratch(c, (size_t)nblk * 9 * sizeof(double))) return rc;
+  if (fp64(c))
+    HIPCHK(Launch<double>::audit(c->L, (const double*)c->U[0], c->cfg.gamma, c->scratch, nblk));
+  else
+    HIPCHK(Launch<float>::audit(c->L, (const float*)c->U[0], c->cfg.gamma, c->scratch, nblk));
+  std::vector<double> part((size_t)nblk * 9);
+  HIPCHK(hipMemcpyAsync(part.data(), c->scratch, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  Ctl h{};
+  HIPCHK(hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nblk; ++b)
+    for (int v = 0; v < 9; ++v) acc[v] = (v == 7) ? std::max(acc[v], part[b * 9 + v]) : acc[v] + part[b * 9 + v];
+  const double dV = c->vol;
+  out->total_entropy = acc[0] * dV;
+  out->kinetic_energy = 0.5 * acc[1] * dV;
+  out->mass = acc[2] * dV;
+  out->momentum[0] = acc[3] * dV;
+  out->momentum[1] = acc[4] * dV;
+  out->momentum[2] = acc[5] * dV;
+  out->energy = acc[6] * dV;
+  out->max_speed = (c->cfg.system == TECNO_SYS_SCALAR && c->cfg.model == TECNO_MODEL_LINADV)
+                       ? c->P64.vel_norm : acc[7];
+  out->n_nonfinite = (int64_t)acc[8];
+  out->n_sign_fallback = (int64_t)h.nfallback;
+  return 0;
+}
+
+int tecno_l1_error(tecno_ctx* c, const double* exact, int64_t n, double* l1) {
+  if (!c || !exact || !l1) return fail(TECNO_EINVAL, "null argument");
+  if (n != c->N) return fail(TECNO_EINVAL, "cell count mismatch");
+  const int nblk = 1024;
+  size_t eb = (size_t)c->N * c->nprim * sizeof(double);
+  if (int rc = ensure_scratch(c, eb + (size_t)nblk * 5 * sizeof(double))) return rc;
+  double* part = c->scratch + (size_t)c->N * c->nprim;
+  HIPCHK(hipMemcpyAsync(c->scratch, exact, eb, hipMemcpyHostToDevice, c->stream));
+  if (fp64(c))
+    HIPCHK(Launch<double>::l1(c->L, (const double*)c->U[0], c->scratch, c->cfg.gamma, part, nblk));
+  else
+    HIPCHK(Launch<float>::l1(c->L, (const float*)c->U[0], c->scratch, c->cfg.gamma, part, nblk));
+  std::vector<double> hp((size_t)nblk * 5);
+  HIPCHK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int v = 0; v < c->nprim; ++v) {
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += hp[b * 5 + v];
+    l1[v] = s * c->vol;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ seams
+template <typename F>
+static int with_device_io(tecno_ctx* c, std::vector<std::pair<const double*, size_t>> in,
+                          std::vector<std::pair<double*, size_t>> out, F launch) {
+  size_t tot = 0;
+  for (auto& p : in) tot += p.second;
+  for (auto& p : out) tot += p.second;
+  if (int rc = ensure_scratch(c, tot * sizeof(double) + 64)) return rc;
+  std::vector<double*> din, dout;
+  double* q = c->scratch;
+  for (auto& p : in) {
+    HIPCHK(hipMemcpyAsync(q, p.first, p.second * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    din.push_back(q);
+    q += p.second;
+  }
+  for (auto& p : out) {
+    dout.push_back(q);
+    q += p.second;
+  }
+  HIPCHK(launch(din, dout));
+  for (size_t i = 0; i < out.size(); ++i)
+    HIPCHK(hipMemcpyAsync(out[i].first, dout[i], out[i].second * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" {
+
+int tecno_reconstruct_batch(tecno_ctx* c, const double* st, int64_t n, double* l, double* r) {
+  if (!c || !st || !l || !r) return fail(TECNO_EINVAL, "null argument");
+  if (n <= 0) return 0;
+  if (int rc = need_net(c)) return rc;
+  const int ng = c->cfg.recon == TECNO_REC_ENO ? c->cfg.eno_level : 2;
+  return with_device_io(c, {{st, (size_t)n * 2 * ng}}, {{l, (size_t)n}, {r, (size_t)n}},
+                        [&](std::vector<double*>& i, std::vector<double*>& o) {
+                          return fp64(c) ? Launch<double>::recon_batch(c->L, c->P64, i[0], ng, n, o[0], o[1])
+                                         : Launch<float>::recon_batch(c->L, c->P32, i[0], ng, n, o[0], o[1]);
+                        });
+}
+
+int tecno_mlp_forward_batch(tecno_ctx* c, const double* u4, int64_t n, double* l, double* r) {
+  if (!c || !u4 || !l || !r) return fail(TECNO_EINVAL, "null argument");
+  if (!c->has_net) return fail(TECNO_ESTATE, "no network loaded");
+  if (n <= 0) return 0;
+  if (c->cfg.mlp_dtype == TECNO_MLP_BF16 || c->cfg.mlp_dtype == TECNO_MLP_FP16)
+    return with_device_io(c, {{u4, (size_t)n * 4}}, {{l, (size_t)n}, {r, (size_t)n}},
+                          [&](std::vector<double*>& i, std::vector<double*>& o) {
+                            return mlp_mfma_batch(c->stream, c->dW32, c->db32, c->net.nl,
+                                                  c->cfg.mlp_dtype == TECNO_MLP_FP16, i[0], n, o[0], o[1]);
+                          });
+  return with_device_io(c, {{u4, (size_t)n * 4}}, {{l, (size_t)n}, {r, (size_t)n}},
+                        [&](std::vector<double*>& i, std::vector<double*>& o) {
+                          return (c->cfg.mlp_dtype == TECNO_MLP_FP64)
+                                     ? Launch<double>::mlp_batch(c->L, c->P64, i[0], n, o[0], o[1])
+                                     : Launch<float>::mlp_batch(c->L, c->P32, i[0], n, o[0], o[1]);
+                        });
+}
+
+int tecno_flux_batch(tecno_ctx* c, const double* P, const double* V, int64_t n, int axis,
+                     double* F) {
+  if (!c || !P || !F) return fail(TECNO_EINVAL, "null argument");
+  if (axis < 0 || axis > 1) return fail(TECNO_EINVAL, "axis must be 0 or 1 for this build");
+  if (n <= 0) return 0;
+  if (int rc = need_net(c)) return rc;
+  const int ng = c->G;
+  const bool sc = c->cfg.system == TECNO_SYS_SCALAR;
+  if (!sc && !V) return fail(TECNO_EINVAL, "entropy variables required");
+  size_t np = sc ? (size_t)n * 2 * ng : (size_t)n * 2 * ng * 5;
+  std::vector<std::pair<const double*, size_t>> in = {{P, np}};
+  if (!sc) in.push_back({V, np});
+  return with_device_io(c, in, {{F, sc ? (size_t)n : (size_t)n * 5}},
+                        [&](std::vector<double*>& i, std::vector<double*>& o) {
+                          const double* vin = sc ? nullptr : i[1];
+                          return fp64(c) ? Launch<double>::flux_batch(c->L, c->P64, i[0], vin, ng, n, axis, o[0])
+                                         : Launch<float>::flux_batch(c->L, c->P32, i[0], vin, ng, n, axis, o[0]);
+                        });
+}
+
+// ------------------------------------------------------------------ timing
+int tecno_timing_enable(tecno_ctx* c, int on) {
+  if (!c) return fail(TECNO_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto& e : c->ev) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  c->ev.clear();
+  c->t_ms_acc = 0;
+  c->t_n = 0;
+  c->timing = on != 0;
+  return 0;
+}
+
+int tecno_timing_get(tecno_ctx* c, double* ms_avg, int64_t* n) {
+  if (!c) return fail(TECNO_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto& e : c->ev) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+    c->t_ms_acc += ms;
+    c->t_n += 1;
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  c->ev.clear();
+  if (ms_avg) *ms_avg = c->t_n ? c->t_ms_acc / c->t_n : 0.0;
+  if (n) *n = c->t_n;
+  return 0;
+}
+
+// ------------------------------------------------------------------ training
+int tecno_train_init(tecno_ctx* c, const tecno_adam* opt) {
+  if (!c || !opt) return fail(TECNO_EINVAL, "null argument");
+  if (!c->has_net) return fail(TECNO_ESTATE, "no network loaded");
+  if (!c->net.nl || c->net.nl > 4) return fail(TECNO_EINVAL, "training supports up to 4 layers");
+  for (int l = 0; l <= c->net.nl; ++l)
+    if (c->net.w[l] > 8) return fail(TECNO_EINVAL, "training supports widths up to 8");
+  delete c->trainer;
+  c->trainer = new MlpTrainer();
+  std::string err;
+  int rc = c->trainer->init(c->stream, c->net.nl, c->net.w, c->net.W.data(), c->net.b.data(),
+                            opt->lr, opt->beta1, opt->beta2, opt->eps, opt->weight_decay, err);
+  if (rc) {
+    delete c->trainer;
+    c->trainer = nullptr;
+    return fail(TECNO_EHIP, err);
+  }
+  return 0;
+}
+
+static int sync_net_from_trainer(tecno_ctx* c) {
+  std::vector<double> W, b;
+  c->trainer->get_weights(W, b);
+  for (int l = 0; l < c->net.nl; ++l)
+    for (int o = 0; o < c->net.w[l + 1]; ++o) {
+      for (int i = 0; i < c->net.w[l]; ++i)
+        c->net.W[(size_t)l * TECNO_NET_MAXW * TECNO_NET_MAXW + o * TECNO_NET_MAXW + i] = W[(size_t)l * 64 + o * 8 + i];
+      c->net.b[(size_t)l * TECNO_NET_MAXW + o] = b[(size_t)l * 8 + o];
+    }
+  return upload_net(c);
+}
+
+int tecno_train_step(tecno_ctx* c, const double* rows, int64_t n, int use_bf16, double* loss) {
+  if (!c || !rows) return fail(TECNO_EINVAL, "null argument");
+  if (!c->trainer) return fail(TECNO_ESTATE, "tecno_train_init first");
+  if (n <= 0) return fail(TECNO_EINVAL, "empty batch");
+  std::string err;
+  double l = 0;
+  if (c->trainer->step(rows, n, use_bf16 != 0, true, &l, nullptr, err)) return fail(TECNO_EHIP, err);
+  if (loss) *loss = l;
+  return sync_net_from_trainer(c);
+}
+
+int tecno_train_grad(tecno_ctx* c, const double* rows, int64_t n, int use_bf16, double* loss,
+                     double* grad) {
+  if (!c || !rows || !grad) return fail(TECNO_EINVAL, "null argument");
+  if (!c->trainer) return fail(TECNO_ESTATE, "tecno_train_init first");
+  if (n <= 0) return fail(TECNO_EINVAL, "empty batch");
+  std::string err;
+  std::vector<double> g;
+  double l = 0;
+  if (c->trainer->step(rows, n, use_bf16 != 0, false, &l, &g, err)) return fail(TECNO_EHIP, err);
+  // trainer layout [l][8x8] / [l][8] -> packed W then b
+  size_t q = 0;
+  for (int L = 0; L < c->net.nl; ++L)
+    for (int o = 0; o < c->net.w[L + 1]; ++o)
+      for (int i = 0; i < c->net.w[L]; ++i) grad[q++] = g[(size_t)L * 64 + o * 8 + i];
+  for (int L = 0; L < c->net.nl; ++L)
+    for (int o = 0; o < c->net.w[L + 1]; ++o) grad[q++] = g[4 * 64 + (size_t)L * 8 + o];
+  if (loss) *loss = l;
+  return 0;
+}
+
+int tecno_make_train_rows(const double* u4, const double* target, int64_t n, double* rows) {
+  if (!u4 || !target || !rows) return fail(TECNO_EINVAL, "null argument");
+  return make_train_rows_host(u4, target, n, rows);
+}
+
+}  // extern "C"

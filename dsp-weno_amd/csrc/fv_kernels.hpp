@@ -1,0 +1,225 @@
+// fv_kernels.hpp — fused SSPRK3 stage kernels for gfx950 and their launchers.
+//
+// One launch advances one Runge-Kutta stage of the whole grid
+// (FV::compute_residual + FV::update_solution, enflo/src/fv.cc:591-767 and
+// 1163-1175; enflo_scalar/src_scalar/fv.cc:317-476).  A workgroup owns a
+// TX x TY tile of cells:
+//   1. it reads the conserved state of the tile and its G-cell cross-shaped
+//      halo from HBM (structure of arrays, x fastest; ghost cells are produced
+//      by index mapping: periodic wrap, Neumann copy, wall mirror), converts
+//      each cell to a record (primitive, entropy variables ...) in LDS;
+//   2. every lane evaluates one face flux per round (x faces and y faces in
+//      one flat, wave-uniform loop) from LDS and keeps it in registers;
+//   3. the fluxes go to LDS (aliasing the records) and each cell accumulates
+//      its residual in the reference's order (-Fx(i-1/2) + Fx(i+1/2)
+//      - Fy(j-1/2) + Fy(j+1/2)), applies the SSPRK3 update and writes back;
+//      the last stage also reduces max(|u| + c) for the next CFL step.
+// Faces on tile edges are evaluated by both neighbouring tiles; the tile
+// shapes are chosen so that the face count fills whole 256-lane rounds.
+#pragma once
+#include "fv_physics.hpp"
+
+namespace tecno {
+
+// ---------------------------------------------------------------- control
+struct Ctl {
+  double dt, time, t_save, t_int, final_time;
+  unsigned long long smax_bits;  // max wave speed of the current state (bits of a double >= 0)
+  int iter, max_iter, active, pad;
+  unsigned long long nbad;       // cells with non-finite or non-positive rho / p
+  unsigned long long nfallback;  // DSP-WENO components replaced by SP-WENO
+};
+
+struct GridDev {
+  int n[3];    // cells held per direction (1 for collapsed axes)
+  int bc[6];
+  int64_t N;
+};
+
+template <typename T>
+struct StageArgs {
+  const T* U;    // U^(k-1), NV x N
+  const T* U0;   // U^n
+  T* Uout;       // U^(k)
+  Ctl* ctl;
+  double a, b;   // u <- b*u + a*u0 - (dt/vol)*b*res
+  int last;      // reduce the CFL wave speed of Uout
+};
+
+// map a (possibly ghost) coordinate along one axis to a source cell; returns
+// the sign to apply to the normal momentum (-1 for a wall mirror)
+__device__ __forceinline__ int bc_map(int c, int n, int bcl, int bcr, int& flip) {
+  flip = 0;
+  if (c < 0) {
+    if (bcl == 0) c += n;                 // periodic
+    else if (bcl == 3) { c = -1 - c; flip = 1; }  // wall: mirror
+    else c = 0;                           // neumann: first interior cell
+  } else if (c >= n) {
+    if (bcr == 0) c -= n;
+    else if (bcr == 3) { c = 2 * n - 1 - c; flip = 1; }
+    else c = n - 1;
+  }
+  // tiles that overhang a small domain: keep every access inside the grid
+  c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+  return c;
+}
+
+// ================================================================ stage kernel
+template <int TX, int TY, int DIMS, int G>
+struct TileShape {
+  static constexpr int BX = TX + 2 * G;
+  static constexpr int BY = (DIMS >= 2) ? TY + 2 * G : 1;
+  static constexpr int NB = BX * BY;
+  static constexpr int NFX = (TX + 1) * TY;
+  static constexpr int NFY = (DIMS >= 2) ? TX * (TY + 1) : 0;
+  static constexpr int NF = NFX + NFY;
+  static constexpr int ROUNDS = (NF + 255) / 256;
+  static constexpr int NCELL = TX * TY;
+};
+
+#ifndef TECNO_HOST_BUILD
+__device__ __forceinline__ double atomic_max_pos(unsigned long long* addr, double v) {
+  // non-negative doubles order like their bit patterns (NaN sorts above +inf)
+  unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return __longlong_as_double((long long)atomicMax(addr, b));
+}
+#endif
+
+// The three phases of a stage on one tile.  They are separate functions so
+// that the same code also runs in a host emulation of the workgroup.
+template <class PH, typename T, int DIMS, int TX, int TY, int G>
+struct Tile {
+  using SH = TileShape<TX, TY, DIMS, G>;
+  static constexpr int NB = SH::NB, NF = SH::NF, NV = PH::NV, NREC = PH::NREC;
+  static constexpr int LDS_N = (NREC * NB > NV * NF) ? NREC * NB : NV * NF;
+  static constexpr int ROUNDS = SH::ROUNDS;
+
+  // 1. cell records of the tile and its cross-shaped halo
+  __device__ static void load(const StageArgs<T>& A, const Phys<T>& P, const GridDev& g, T* lds,
+                              int tid, int bxi, int byi) {
+    const int x0 = bxi * TX, y0 = byi * TY;
+    for (int c = tid; c < NB; c += 256) {
+      const int bx = c % SH::BX, by = c / SH::BX;
+      const int lx = bx - G, ly = (DIMS >= 2) ? by - G : 0;
+      const bool inx = (lx >= 0 && lx < TX), iny = (ly >= 0 && ly < TY);
+      if (!inx && !iny) continue;  // corner of the box: not in any stencil
+      int fx, fy = 0;
+      const int sx = bc_map(x0 + lx, g.n[0], g.bc[0], g.bc[1], fx);
+      int sy = 0;
+      if (DIMS >= 2) sy = bc_map(y0 + ly, g.n[1], g.bc[2], g.bc[3], fy);
+      const int64_t idx = (int64_t)sy * g.n[0] + sx;
+      T rec[NREC];
+      PH::load(P, A.U, g.N, idx, fx, fy, rec);
+#pragma unroll
+      for (int r = 0; r < NREC; ++r) lds[r * NB + c] = rec[r];
+    }
+  }
+
+  // 2. face fluxes, one face per lane per round, kept in registers
+  __device__ static void faces(const Phys<T>& P, const T* lds, int tid, T Fr[ROUNDS][NV]) {
+#pragma unroll
+    for (int rd = 0; rd < ROUNDS; ++rd) {
+      int f = rd * 256 + tid;
+      if (rd * 256 + (tid & ~63) >= NF) break;  // whole wave past the end
+      f = f < NF ? f : NF - 1;
+      int base, stride, ax;
+      if (f < SH::NFX) {
+        const int fy = f / (TX + 1), fx = f - fy * (TX + 1);
+        base = (DIMS >= 2 ? (fy + G) * SH::BX : 0) + fx;
+        stride = 1;
+        ax = 0;
+      } else {
+        const int q = f - SH::NFX;
+        const int fy = q / TX, fx = q - fy * TX;
+        base = fy * SH::BX + fx + G;
+        stride = SH::BX;
+        ax = 1;
+      }
+      PH::face(P, lds, NB, base, stride, ax, Fr[rd]);
+    }
+  }
+
+  // face fluxes -> LDS (after the records are dead)
+  __device__ static void store(T* lds, int tid, const T Fr[ROUNDS][NV]) {
+#pragma unroll
+    for (int rd = 0; rd < ROUNDS; ++rd) {
+      const int f = rd * 256 + tid;
+      if (f < NF) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lds[v * NF + f] = Fr[rd][v];
+      }
+    }
+  }
+
+  // 3. residual in the reference's order, SSPRK3 update, CFL speed
+  __device__ static void update(const StageArgs<T>& A, const Phys<T>& P, const GridDev& g,
+                                const T* lds, int tid, int bxi, int byi, double dt, T& smax,
+                                unsigned& nbad) {
+    const int x0 = bxi * TX, y0 = byi * TY;
+    const int64_t N = g.N;
+    const double factor = dt / P.vol_d;
+    const T a = T(A.a), b = T(A.b), fb = T(factor * A.b);
+    for (int c = tid; c < SH::NCELL; c += 256) {
+      const int ly = c / TX, lx = c - ly * TX;
+      const int gx = x0 + lx, gy = y0 + ly;
+      if (gx >= g.n[0] || (DIMS >= 2 && gy >= g.n[1])) continue;
+      const int64_t idx = (int64_t)gy * g.n[0] + gx;
+      const int fxl = ly * (TX + 1) + lx;
+      const int fyl = SH::NFX + ly * TX + lx;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        T r = -lds[v * NF + fxl];
+        r += lds[v * NF + fxl + 1];
+        if (DIMS >= 2) {
+          r -= lds[v * NF + fyl];
+          r += lds[v * NF + fyl + TX];
+        }
+        const T u = A.U[v * N + idx];
+        const T u0 = A.U0[v * N + idx];
+        T un = u * b;
+        un += (u0 * a - fb * r);
+        A.Uout[v * N + idx] = un;
+      }
+      if (A.last) {
+        smax = m_max(smax, PH::speed(P, A.Uout, N, idx));
+        nbad += PH::bad(A.Uout, N, idx) ? 1u : 0u;
+      }
+    }
+  }
+};
+
+#ifndef TECNO_HOST_BUILD
+template <class PH, typename T, int DIMS, int TX, int TY, int G>
+__global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> A, Phys<T> P, GridDev g) {
+  using TL = Tile<PH, T, DIMS, TX, TY, G>;
+  __shared__ T lds[TL::LDS_N];
+  __shared__ double red[4];
+  if (!A.ctl->active) return;
+  const int tid = threadIdx.x;
+  TL::load(A, P, g, lds, tid, blockIdx.x, blockIdx.y);
+  __syncthreads();
+  T Fr[TL::ROUNDS][TL::NV];
+  TL::faces(P, lds, tid, Fr);
+  __syncthreads();
+  TL::store(lds, tid, Fr);
+  __syncthreads();
+  T smax = T(0);
+  unsigned nbad = 0;
+  TL::update(A, P, g, lds, tid, blockIdx.x, blockIdx.y, A.ctl->dt, smax, nbad);
+  if (A.last) {
+    // wave max, then across the 4 waves
+    double s = (double)smax;
+    for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off));
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+      double m = red[0];
+      for (int w = 1; w < 4; ++w) m = fmax(m, red[w]);
+      atomic_max_pos(&A.ctl->smax_bits, m);
+    }
+    if (nbad) atomicAdd(&A.ctl->nbad, (unsigned long long)nbad);
+  }
+}
+#endif
+
+}  // namespace tecno

@@ -1,0 +1,414 @@
+// fv_launch.hpp — host launchers and the small kernels around the stage
+// kernel: time-step control, state conversion, audit sums, batched seams.
+// Included by one translation unit per arithmetic type (kern_fp64.hip,
+// kern_fp32.hip), which instantiate Launch<T>.
+#pragma once
+#include "fv_kernels.hpp"
+
+namespace tecno {
+
+struct LaunchInfo {
+  int system;     // 0 scalar, 1 euler
+  int dims;       // 1, 2, 3
+  int G;          // halo width (2 or 3)
+  int fast_flux;  // FV_* id for the 2D Euler kernel
+  int fast_rec;   // RV_* id
+  GridDev grid;
+  hipStream_t stream;
+  Ctl* ctl;
+};
+
+// ---------------------------------------------------------------- control
+hipError_t launch_dt(const LaunchInfo& L, double hmin, double cfl, int const_speed, double speed);
+hipError_t launch_finish(const LaunchInfo& L);
+
+#ifdef TECNO_DEFINE_CONTROL
+// dt of the next step (FV::compute_dt, enflo/src/fv.cc:366-383, and the
+// output-stamp clamp of FV::run, fv.cc:1262-1266)
+__global__ void k_dt(Ctl* c, double hmin, double cfl, int const_speed, double speed) {
+  if (!(c->time < c->final_time && c->iter < c->max_iter)) {
+    c->active = 0;
+    c->dt = 0.0;
+    return;
+  }
+  c->active = 1;
+  double smax = const_speed ? speed : __longlong_as_double((long long)c->smax_bits);
+  double dt = 1.0e20;
+  double q = hmin / smax;
+  dt = (q < dt) ? q : dt;
+  dt *= cfl;
+  if (c->time + dt > c->t_save) dt = c->t_save - c->time;
+  c->dt = dt;
+  c->smax_bits = 0ull;
+}
+
+__global__ void k_finish(Ctl* c) {
+  if (!c->active) return;
+  c->time += c->dt;
+  c->iter += 1;
+  if (c->time == c->t_save) c->t_save += c->t_int;
+}
+
+hipError_t launch_dt(const LaunchInfo& L, double hmin, double cfl, int const_speed, double speed) {
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, L.stream, L.ctl, hmin, cfl, const_speed, speed);
+  return hipGetLastError();
+}
+hipError_t launch_finish(const LaunchInfo& L) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, L.stream, L.ctl);
+  return hipGetLastError();
+}
+#endif
+
+// ---------------------------------------------------------------- conversions
+// prim_to_con (enflo/src/fv.cc:405-416); scalar: copy
+template <typename T>
+__global__ void k_prim_to_con(int system, int dims, int64_t N, const double* prim, T* U,
+                              double gamma) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (system == 0) {
+    U[i] = T(prim[i]);
+    return;
+  }
+  const int nv = dims == 3 ? 5 : 4;  // primitive: rho, u, v, [w,] p
+  double rho = prim[i], p = prim[(nv - 1) * N + i];
+  double u[3] = {prim[N + i], prim[2 * N + i], dims == 3 ? prim[3 * N + i] : 0.0};
+  U[i] = T(rho);
+  for (int d = 0; d < nv - 2; ++d) U[(1 + d) * N + i] = T(u[d] * rho);
+  double q = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  U[(nv - 1) * N + i] = T(0.5 * rho * q + p / (gamma - 1.0));
+}
+
+// con_to_prim (enflo/src/fv.cc:388-400)
+template <typename T>
+__global__ void k_con_to_prim(int system, int dims, int64_t N, const T* U, double* prim,
+                              double gamma) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  if (system == 0) {
+    prim[i] = double(U[i]);
+    return;
+  }
+  const int nv = dims == 3 ? 5 : 4;
+  T rho = U[i];
+  T inv = T(1) / rho;
+  T m2 = T(0);
+  for (int d = 0; d < nv - 2; ++d) {
+    T m = U[(1 + d) * N + i];
+    prim[(1 + d) * N + i] = double(m * inv);
+    m2 += m * m;
+  }
+  prim[i] = double(rho);
+  T p = T(gamma - 1.0) * (U[(nv - 1) * N + i] - T(0.5) * m2 / rho);
+  prim[(nv - 1) * N + i] = double(p);
+}
+
+// max wave speed of a state (first step's dt)
+template <class PH, typename T>
+__global__ void k_speed(Phys<T> P, int64_t N, const T* U, Ctl* c) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s = fmax(s, (double)PH::speed(P, U, N, i));
+  for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    atomic_max_pos(&c->smax_bits, m);
+  }
+}
+
+// FP64 audit sums (compute_global, enflo/src/compute_global.cc:10-89):
+// out[0] sum -rho*s/(gamma-1) (scalar: sum u^2/2), out[1] sum rho|u|^2,
+// out[2..6] sums of the conserved variables, out[7] max speed, out[8] #bad.
+template <typename T>
+__global__ void k_audit(int system, int dims, int64_t N, const T* U, double gamma,
+                        double* part) {
+  __shared__ double red[4][9];
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (system == 0) {
+      double u = (double)U[i];
+      acc[0] += 0.5 * u * u;
+      acc[2] += u;
+      acc[7] = fmax(acc[7], fabs(u));
+      if (!(u == u)) acc[8] += 1;
+    } else {
+      const int nv = dims == 3 ? 5 : 4;
+      double rho = U[i], m[3] = {U[N + i], U[2 * N + i], dims == 3 ? (double)U[3 * N + i] : 0.0};
+      double E = U[(nv - 1) * N + i];
+      double inv = 1.0 / rho;
+      double u[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
+      double q = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+      double p = (gamma - 1.0) * (E - 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / rho);
+      double s = log(p) - gamma * log(rho);
+      acc[0] += -rho * s / (gamma - 1.0);
+      acc[1] += rho * q;
+      acc[2] += rho;
+      acc[3] += m[0];
+      acc[4] += m[1];
+      acc[5] += m[2];
+      acc[6] += E;
+      acc[7] = fmax(acc[7], sqrt(q) + sqrt(gamma * p / rho));
+      if (!(rho > 0.0) || !(p > 0.0) || !(p < INFINITY)) acc[8] += 1;
+    }
+  }
+  for (int v = 0; v < 9; ++v) {
+    double x = acc[v];
+    for (int off = 32; off > 0; off >>= 1) {
+      double y = __shfl_xor(x, off);
+      x = (v == 7) ? fmax(x, y) : x + y;
+    }
+    acc[v] = x;
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int v = 0; v < 9; ++v) red[threadIdx.x >> 6][v] = acc[v];
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    int v = threadIdx.x;
+    double x = red[0][v];
+    for (int w = 1; w < 4; ++w) x = (v == 7) ? fmax(x, red[w][v]) : x + red[w][v];
+    part[(int64_t)blockIdx.x * 9 + v] = x;
+  }
+}
+
+// L1 error (compute_error, enflo/src/compute_errors.cc:9-34): per-block sums
+// of |exact - prim| for each primitive variable
+template <typename T>
+__global__ void k_l1(int system, int dims, int64_t N, const T* U, const double* exact,
+                     double gamma, double* part) {
+  __shared__ double red[4][5];
+  const int nv = system == 0 ? 1 : (dims == 3 ? 5 : 4);
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double pr[5];
+    if (system == 0) {
+      pr[0] = U[i];
+    } else {
+      double rho = U[i];
+      double inv = 1.0 / rho, m2 = 0.0;
+      for (int d = 0; d < nv - 2; ++d) {
+        double m = U[(1 + d) * N + i];
+        pr[1 + d] = m * inv;
+        m2 += m * m;
+      }
+      pr[0] = rho;
+      pr[nv - 1] = (gamma - 1.0) * (U[(nv - 1) * N + i] - 0.5 * m2 / rho);
+    }
+    for (int v = 0; v < nv; ++v) acc[v] += fabs(exact[v * N + i] - pr[v]);
+  }
+  for (int v = 0; v < 5; ++v) {
+    double x = acc[v];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    acc[v] = x;
+  }
+  if ((threadIdx.x & 63) == 0)
+    for (int v = 0; v < 5; ++v) red[threadIdx.x >> 6][v] = acc[v];
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    int v = threadIdx.x;
+    part[(int64_t)blockIdx.x * 5 + v] = red[0][v] + red[1][v] + red[2][v] + red[3][v];
+  }
+}
+
+// ---------------------------------------------------------------- seams
+template <typename T>
+__global__ void k_recon_batch(Phys<T> P, const double* st, int ng, int64_t n, double* l,
+                              double* r) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T s[6];
+  for (int k = 0; k < 2 * ng; ++k) s[k] = T(st[i * 2 * ng + k]);
+  T a, b;
+  recon_any(P, s, ng, a, b);
+  l[i] = a;
+  r[i] = b;
+}
+
+template <typename T>
+__global__ void k_mlp_batch(Phys<T> P, const double* u4, int64_t n, double* l, double* r) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T a, b;
+  nn_forward(P.net, P.arch5 != 0, T(u4[4 * i]), T(u4[4 * i + 1]), T(u4[4 * i + 2]),
+             T(u4[4 * i + 3]), a, b);
+  l[i] = a;
+  r[i] = b;
+}
+
+// FV::num_flux on independent faces.  Scalar: P = n x 2ng.  2D Euler: P, V
+// = n x 2ng x 5 (w and Vz must be zero), F = n x 5.
+template <typename T, int G>
+__global__ void k_flux_batch(Phys<T> P, int system, const double* Pin, const double* Vin,
+                             int ng, int64_t n, int ax, double* F) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  constexpr int S = 2 * G;
+  if (system == 0) {
+    T loc[S];
+    for (int k = 0; k < S; ++k) loc[k] = T(Pin[i * 2 * ng + k]);
+    T f[1];
+    ScalarPh<T, G>::face(P, loc, S, 0, 1, ax, f);
+    F[i] = f[0];
+    return;
+  }
+  T loc[E_NREC * S];
+  for (int k = 0; k < S; ++k) {
+    const double* q = Pin + (i * S + k) * 5;
+    const double* v = Vin + (i * S + k) * 5;
+    T rho = q[0], p = q[4];
+    loc[E_RHO * S + k] = rho;
+    loc[E_UX * S + k] = T(q[1]);
+    loc[E_UY * S + k] = T(q[2]);
+    loc[E_P * S + k] = p;
+    loc[E_BK * S + k] = T(0.5) / (P.R * (p / (P.R * rho)));
+    loc[E_VF * S + k] = T(v[0]);
+    loc[E_VX * S + k] = T(v[1]);
+    loc[E_VY * S + k] = T(v[2]);
+    loc[E_VL * S + k] = T(v[4]);
+  }
+  T f[4];
+  Euler2Ph<T, G, FV_GENERIC, RV_GENERIC>::face(P, loc, S, 0, 1, ax, f);
+  F[5 * i + 0] = f[0];
+  F[5 * i + 1] = f[1];
+  F[5 * i + 2] = f[2];
+  F[5 * i + 3] = 0.0;
+  F[5 * i + 4] = f[3];
+}
+
+// ---------------------------------------------------------------- launchers
+template <typename T>
+struct Launch {
+  static hipError_t stage(const LaunchInfo& L, const Phys<T>& P, const T* U, const T* U0, T* Uout,
+                          double a, double b, int last);
+  static hipError_t speed(const LaunchInfo& L, const Phys<T>& P, const T* U);
+  static hipError_t prim_to_con(const LaunchInfo& L, const double* prim, T* U, double gamma);
+  static hipError_t con_to_prim(const LaunchInfo& L, const T* U, double* prim, double gamma);
+  static hipError_t audit(const LaunchInfo& L, const T* U, double gamma, double* part, int nblk);
+  static hipError_t l1(const LaunchInfo& L, const T* U, const double* exact, double gamma,
+                       double* part, int nblk);
+  static hipError_t recon_batch(const LaunchInfo& L, const Phys<T>& P, const double* st, int ng,
+                                int64_t n, double* l, double* r);
+  static hipError_t mlp_batch(const LaunchInfo& L, const Phys<T>& P, const double* u4, int64_t n,
+                              double* l, double* r);
+  static hipError_t flux_batch(const LaunchInfo& L, const Phys<T>& P, const double* Pin,
+                               const double* Vin, int ng, int64_t n, int ax, double* F);
+};
+
+#define TECNO_TILE2_X 32
+#define TECNO_TILE2_Y 15
+#define TECNO_TILE1_X 255
+
+template <class PH, typename T, int DIMS, int TX, int TY, int G>
+static hipError_t launch_stage_t(const LaunchInfo& L, const Phys<T>& P, const StageArgs<T>& A) {
+  dim3 grid((L.grid.n[0] + TX - 1) / TX, DIMS >= 2 ? (L.grid.n[1] + TY - 1) / TY : 1, 1);
+  hipLaunchKernelGGL((stage_kernel<PH, T, DIMS, TX, TY, G>), grid, dim3(256), 0, L.stream, A, P,
+                     L.grid);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const T* U, const T* U0,
+                            T* Uout, double a, double b, int last) {
+  StageArgs<T> A{U, U0, Uout, L.ctl, a, b, last};
+  constexpr int X2 = TECNO_TILE2_X, Y2 = TECNO_TILE2_Y, X1 = TECNO_TILE1_X;
+  if (L.system == 0) {
+    if (L.dims == 1)
+      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, T, 1, X1, 1, 3>(L, P, A)
+                      : launch_stage_t<ScalarPh<T, 2>, T, 1, X1, 1, 2>(L, P, A);
+    if (L.dims == 2)
+      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, T, 2, X2, Y2, 3>(L, P, A)
+                      : launch_stage_t<ScalarPh<T, 2>, T, 2, X2, Y2, 2>(L, P, A);
+    return hipErrorNotSupported;
+  }
+  if (L.dims == 2) {
+    if (L.G == 3)
+      return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_GENERIC>, T, 2, X2, Y2, 3>(L, P, A);
+    if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_SPWENO)
+      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_SPWENO>, T, 2, X2, Y2, 2>(L, P, A);
+    if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_DSP)
+      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_DSP>, T, 2, X2, Y2, 2>(L, P, A);
+    return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, T, 2, X2, Y2, 2>(L, P, A);
+  }
+  return hipErrorNotSupported;
+}
+
+template <typename T>
+hipError_t Launch<T>::speed(const LaunchInfo& L, const Phys<T>& P, const T* U) {
+  int nblk = (int)((L.grid.N + 255) / 256);
+  if (nblk > 4096) nblk = 4096;
+  if (L.system == 0)
+    hipLaunchKernelGGL((k_speed<ScalarPh<T, 2>, T>), dim3(nblk), dim3(256), 0, L.stream, P,
+                       L.grid.N, U, L.ctl);
+  else
+    hipLaunchKernelGGL((k_speed<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, T>), dim3(nblk), dim3(256),
+                       0, L.stream, P, L.grid.N, U, L.ctl);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::prim_to_con(const LaunchInfo& L, const double* prim, T* U, double gamma) {
+  int nblk = (int)((L.grid.N + 255) / 256);
+  hipLaunchKernelGGL((k_prim_to_con<T>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims,
+                     L.grid.N, prim, U, gamma);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::con_to_prim(const LaunchInfo& L, const T* U, double* prim, double gamma) {
+  int nblk = (int)((L.grid.N + 255) / 256);
+  hipLaunchKernelGGL((k_con_to_prim<T>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims,
+                     L.grid.N, U, prim, gamma);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::audit(const LaunchInfo& L, const T* U, double gamma, double* part,
+                            int nblk) {
+  hipLaunchKernelGGL((k_audit<T>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims, L.grid.N,
+                     U, gamma, part);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::l1(const LaunchInfo& L, const T* U, const double* exact, double gamma,
+                         double* part, int nblk) {
+  hipLaunchKernelGGL((k_l1<T>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims, L.grid.N, U,
+                     exact, gamma, part);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::recon_batch(const LaunchInfo& L, const Phys<T>& P, const double* st, int ng,
+                                  int64_t n, double* l, double* r) {
+  hipLaunchKernelGGL((k_recon_batch<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     L.stream, P, st, ng, n, l, r);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::mlp_batch(const LaunchInfo& L, const Phys<T>& P, const double* u4, int64_t n,
+                                double* l, double* r) {
+  hipLaunchKernelGGL((k_mlp_batch<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, L.stream,
+                     P, u4, n, l, r);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::flux_batch(const LaunchInfo& L, const Phys<T>& P, const double* Pin,
+                                 const double* Vin, int ng, int64_t n, int ax, double* F) {
+  dim3 gr((unsigned)((n + 255) / 256));
+  if (ng == 3)
+    hipLaunchKernelGGL((k_flux_batch<T, 3>), gr, dim3(256), 0, L.stream, P, L.system, Pin, Vin,
+                       ng, n, ax, F);
+  else
+    hipLaunchKernelGGL((k_flux_batch<T, 2>), gr, dim3(256), 0, L.stream, P, L.system, Pin, Vin,
+                       ng, n, ax, F);
+  return hipGetLastError();
+}
+
+}  // namespace tecno

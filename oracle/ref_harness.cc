@@ -17,11 +17,14 @@
 //          scalar:  in = n x 2ng, out = n x 1.
 //   ent   <param.in> <n> <in.bin> <out.bin>  prim_to_ent on n primitive
 //          states (systems only; via the entropy_var array).
+//   time  <param.in> <nsteps>   runs nsteps steps of the time loop and prints
+//          the wall time of the steps alone (initialisation excluded).
 
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -144,6 +147,50 @@ static int mode_run(char* param, int nsteps, const char* out) {
   return 0;
 }
 
+static int mode_time(char* param, int nsteps) {
+  FV fv(param);
+  setup(fv);
+#ifndef HARNESS_SCALAR
+  fv.prim_to_con();
+  fv.prim_to_ent();
+#endif
+  fv.time = 0.0;
+  fv.iter = 0;
+  double t_int = fv.param.final_time / fv.param.t_stamps;
+  double t_save = t_int;
+  struct timespec a, b;
+  clock_gettime(CLOCK_MONOTONIC, &a);
+  for (int s = 0; s < nsteps; ++s) {
+#ifdef HARNESS_SCALAR
+    fv.solution_old = fv.solution;
+#else
+    fv.conserved_old = fv.conserved;
+#endif
+    fv.compute_dt();
+    if (fv.time + fv.dt > t_save) fv.dt = t_save - fv.time;
+    for (int rk = 0; rk < fv.param.n_rks; ++rk) {
+#ifdef HARNESS_SCALAR
+      fv.compute_residual(rk);
+      fv.update_solution(rk);
+      fv.apply_bc();
+#else
+      fv.prim_to_ent();
+      fv.compute_residual(rk);
+      fv.update_solution(rk);
+      fv.con_to_prim();
+#endif
+    }
+    fv.time += fv.dt;
+    ++fv.iter;
+    if (fv.time == t_save) t_save += t_int;
+  }
+  clock_gettime(CLOCK_MONOTONIC, &b);
+  double sec = (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
+  fprintf(stdout, "steps %d seconds %.6f cells %lld\n", nsteps, sec,
+          (long long)fv.Nx_cell * fv.Ny_cell * fv.Nz_cell);
+  return 0;
+}
+
 static int mode_recon(char* param, int n, int ng, const char* in, const char* out) {
   FV fv(param);
   setup(fv);
@@ -246,6 +293,7 @@ int main(int argc, char** argv) {
   }
   std::cout.setstate(std::ios_base::failbit); /* silence the solver's log */
   std::string m = argv[1];
+  if (m == "time" && argc == 4) return mode_time(argv[2], atoi(argv[3]));
   if (m == "run" && argc == 5) return mode_run(argv[2], atoi(argv[3]), argv[4]);
   if (m == "recon" && argc == 7)
     return mode_recon(argv[2], atoi(argv[3]), atoi(argv[4]), argv[5], argv[6]);

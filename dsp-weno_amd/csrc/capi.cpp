@@ -17,6 +17,8 @@
 namespace tecno {
 extern template struct Launch<double>;
 extern template struct Launch<float>;
+extern template struct StateOps<double>;
+extern template struct StateOps<float>;
 }  // namespace tecno
 
 using namespace tecno;
@@ -119,6 +121,9 @@ static void fill_phys(const tecno_ctx* c, Phys<T>& P) {
   P.area[0] = T(c->h[1] * c->h[2]);
   P.area[1] = T(c->h[0] * c->h[2]);
   P.area[2] = T(c->h[0] * c->h[1]);
+  P.area_d[0] = c->h[1] * c->h[2];
+  P.area_d[1] = c->h[0] * c->h[2];
+  P.area_d[2] = c->h[0] * c->h[1];
   P.hmin = T(c->hmin);
   P.cfl = T(g.cfl);
   P.vol = T(c->vol);
@@ -128,6 +133,14 @@ static void fill_phys(const tecno_ctx* c, Phys<T>& P) {
   P.s05 = T(std::sqrt(0.5));
   P.sgm1 = T(std::sqrt(g.gamma - 1.0));
   P.gpgm = T((g.gamma + 1.0) / (g.gamma - 1.0));
+  P.inv_gamma = T(1.0 / g.gamma);
+  P.gamma_d = g.gamma;
+  P.gm1_d = g.gamma - 1.0;
+  P.inv_gm1_d = 1.0 / (g.gamma - 1.0);
+  P.R_d = g.gas_const;
+  P.s05_d = std::sqrt(0.5);
+  P.sgm1_d = std::sqrt(g.gamma - 1.0);
+  P.inv_gamma_d = 1.0 / g.gamma;
   P.vel_norm = std::sqrt(g.vel[0] * g.vel[0] + g.vel[1] * g.vel[1] + g.vel[2] * g.vel[2]);
   P.arch5 = 0;
   P.net.nl = c->net.nl;
@@ -179,11 +192,17 @@ static bool fp64(const tecno_ctx* c) { return c->cfg.precision == TECNO_FP64; }
 
 static hipError_t launch_stage(tecno_ctx* c, const void* U, const void* U0, void* Uo, double a,
                                double b, int last) {
-  if (fp64(c))
-    return Launch<double>::stage(c->L, c->P64, (const double*)U, (const double*)U0, (double*)Uo, a,
-                                 b, last);
-  return Launch<float>::stage(c->L, c->P32, (const float*)U, (const float*)U0, (float*)Uo, a, b,
-                              last);
+  if (fp64(c)) return Launch<double>::stage(c->L, c->P64, U, U0, Uo, a, b, last);
+  return Launch<float>::stage(c->L, c->P32, U, U0, Uo, a, b, last);
+}
+
+static hipError_t st_prim_to_con(tecno_ctx* c, const double* prim, void* U) {
+  return c->L.state_fp64 ? StateOps<double>::prim_to_con(c->L, prim, U, c->cfg.gamma)
+                         : StateOps<float>::prim_to_con(c->L, prim, U, c->cfg.gamma);
+}
+static hipError_t st_con_to_prim(tecno_ctx* c, const void* U, double* prim) {
+  return c->L.state_fp64 ? StateOps<double>::con_to_prim(c->L, U, prim, c->cfg.gamma)
+                         : StateOps<float>::con_to_prim(c->L, U, prim, c->cfg.gamma);
 }
 
 // ------------------------------------------------------------------ API
@@ -265,7 +284,16 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
       delete c;
       return fail(TECNO_EINVAL, "grid too small for the stencil");
     }
-  c->esize = fp64(c) ? 8 : 4;
+  // the mixed-precision Euler path keeps the state in FP64
+  c->L.state_fp64 = (fp64(c) || g.system == TECNO_SYS_EULER) ? 1 : 0;
+  c->esize = c->L.state_fp64 ? 8 : 4;
+  if (!fp64(c) && g.system == TECNO_SYS_EULER &&
+      (g.flux != TECNO_FLUX_KEPES_TECNO4_ROE || g.ec1_fix || g.recon == TECNO_REC_ENO ||
+       g.recon == TECNO_REC_FIRST || g.spweno_corr == 2 && g.recon != TECNO_REC_SP_WENO)) {
+    delete c;
+    return fail(TECNO_EINVAL, "FP32 Euler path: kepes_tecno4_roe with sp_weno, sp_wenoc or "
+                              "sp_weno_dl only (use FP64 for the other options)");
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(TECNO_EHIP, "hipStreamCreate failed");
@@ -443,10 +471,7 @@ int tecno_set_state(tecno_ctx* c, const double* prim, int64_t n) {
   size_t bytes = (size_t)c->N * c->nprim * sizeof(double);
   if (int rc = ensure_scratch(c, bytes)) return rc;
   HIPCHK(hipMemcpyAsync(c->scratch, prim, bytes, hipMemcpyHostToDevice, c->stream));
-  if (fp64(c))
-    HIPCHK(Launch<double>::prim_to_con(c->L, c->scratch, (double*)c->U[0], c->cfg.gamma));
-  else
-    HIPCHK(Launch<float>::prim_to_con(c->L, c->scratch, (float*)c->U[0], c->cfg.gamma));
+  HIPCHK(st_prim_to_con(c, c->scratch, c->U[0]));
   Ctl h{};
   h.time = 0.0;
   h.iter = 0;
@@ -457,9 +482,9 @@ int tecno_set_state(tecno_ctx* c, const double* prim, int64_t n) {
   HIPCHK(hipMemcpyAsync(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
   if (!(c->cfg.system == TECNO_SYS_SCALAR && c->cfg.model == TECNO_MODEL_LINADV)) {
     if (fp64(c))
-      HIPCHK(Launch<double>::speed(c->L, c->P64, (const double*)c->U[0]));
+      HIPCHK(Launch<double>::speed(c->L, c->P64, c->U[0]));
     else
-      HIPCHK(Launch<float>::speed(c->L, c->P32, (const float*)c->U[0]));
+      HIPCHK(Launch<float>::speed(c->L, c->P32, c->U[0]));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   c->state_set = true;
@@ -473,10 +498,7 @@ int tecno_get_state(tecno_ctx* c, double* prim, int64_t n) {
   if (!c->state_set) return fail(TECNO_ESTATE, "no state");
   size_t bytes = (size_t)c->N * c->nprim * sizeof(double);
   if (int rc = ensure_scratch(c, bytes)) return rc;
-  if (fp64(c))
-    HIPCHK(Launch<double>::con_to_prim(c->L, (const double*)c->U[0], c->scratch, c->cfg.gamma));
-  else
-    HIPCHK(Launch<float>::con_to_prim(c->L, (const float*)c->U[0], c->scratch, c->cfg.gamma));
+  HIPCHK(st_con_to_prim(c, c->U[0], c->scratch));
   HIPCHK(hipMemcpyAsync(prim, c->scratch, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
@@ -606,10 +628,10 @@ int tecno_audit_state(tecno_ctx* c, tecno_audit* out) {
   if (!c->state_set) return fail(TECNO_ESTATE, "no state");
   const int nblk = 1024;
   if (int rc = ensure_scratch(c, (size_t)nblk * 9 * sizeof(double))) return rc;
-  if (fp64(c))
-    HIPCHK(Launch<double>::audit(c->L, (const double*)c->U[0], c->cfg.gamma, c->scratch, nblk));
+  if (c->L.state_fp64)
+    HIPCHK(StateOps<double>::audit(c->L, c->U[0], c->cfg.gamma, c->scratch, nblk));
   else
-    HIPCHK(Launch<float>::audit(c->L, (const float*)c->U[0], c->cfg.gamma, c->scratch, nblk));
+    HIPCHK(StateOps<float>::audit(c->L, c->U[0], c->cfg.gamma, c->scratch, nblk));
   std::vector<double> part((size_t)nblk * 9);
   HIPCHK(hipMemcpyAsync(part.data(), c->scratch, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   Ctl h{};
@@ -641,10 +663,10 @@ int tecno_l1_error(tecno_ctx* c, const double* exact, int64_t n, double* l1) {
   if (int rc = ensure_scratch(c, eb + (size_t)nblk * 5 * sizeof(double))) return rc;
   double* part = c->scratch + (size_t)c->N * c->nprim;
   HIPCHK(hipMemcpyAsync(c->scratch, exact, eb, hipMemcpyHostToDevice, c->stream));
-  if (fp64(c))
-    HIPCHK(Launch<double>::l1(c->L, (const double*)c->U[0], c->scratch, c->cfg.gamma, part, nblk));
+  if (c->L.state_fp64)
+    HIPCHK(StateOps<double>::l1(c->L, c->U[0], c->scratch, c->cfg.gamma, part, nblk));
   else
-    HIPCHK(Launch<float>::l1(c->L, (const float*)c->U[0], c->scratch, c->cfg.gamma, part, nblk));
+    HIPCHK(StateOps<float>::l1(c->L, c->U[0], c->scratch, c->cfg.gamma, part, nblk));
   std::vector<double> hp((size_t)nblk * 5);
   HIPCHK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

@@ -365,6 +365,10 @@ struct Phys {
   T h[3];
   T hmin, cfl, vol;
   T s05, sgm1, gpgm;  // sqrt(0.5), sqrt(gamma-1), (gamma+1)/(gamma-1)
+  T inv_gamma;
+  double gamma_d, gm1_d, inv_gm1_d, R_d;
+  double area_d[3];
+  double s05_d, sgm1_d, inv_gamma_d;
   double vol_d;       // dx*dy*dz (param.in spacings)
   double hmin_d, cfl_d;
   double vel_norm;    // |vel| (dt of linear advection)

@@ -87,16 +87,21 @@ __device__ __forceinline__ double atomic_max_pos(unsigned long long* addr, doubl
 
 // The three phases of a stage on one tile.  They are separate functions so
 // that the same code also runs in a host emulation of the workgroup.
-template <class PH, typename T, int DIMS, int TX, int TY, int G>
+template <class PH, int DIMS, int TX, int TY, int G>
 struct Tile {
   using SH = TileShape<TX, TY, DIMS, G>;
-  static constexpr int NB = SH::NB, NF = SH::NF, NV = PH::NV, NREC = PH::NREC;
-  static constexpr int LDS_N = (NREC * NB > NV * NF) ? NREC * NB : NV * NF;
+  using TS = typename PH::TS;
+  using TF = typename PH::TF;
+  using TP = typename PH::TP;
+  static constexpr int NB = SH::NB, NF = SH::NF, NV = PH::NV;
+  static constexpr size_t REC_BYTES = PH::rec_bytes(NB);
+  static constexpr size_t FLUX_BYTES = (size_t)NV * NF * sizeof(TF);
+  static constexpr size_t LDS_BYTES = REC_BYTES > FLUX_BYTES ? REC_BYTES : FLUX_BYTES;
   static constexpr int ROUNDS = SH::ROUNDS;
 
   // 1. cell records of the tile and its cross-shaped halo
-  __device__ static void load(const StageArgs<T>& A, const Phys<T>& P, const GridDev& g, T* lds,
-                              int tid, int bxi, int byi) {
+  __device__ static void load(const StageArgs<TS>& A, const Phys<TP>& P, const GridDev& g,
+                              void* lds, int tid, int bxi, int byi) {
     const int x0 = bxi * TX, y0 = byi * TY;
     for (int c = tid; c < NB; c += 256) {
       const int bx = c % SH::BX, by = c / SH::BX;
@@ -108,15 +113,12 @@ struct Tile {
       int sy = 0;
       if (DIMS >= 2) sy = bc_map(y0 + ly, g.n[1], g.bc[2], g.bc[3], fy);
       const int64_t idx = (int64_t)sy * g.n[0] + sx;
-      T rec[NREC];
-      PH::load(P, A.U, g.N, idx, fx, fy, rec);
-#pragma unroll
-      for (int r = 0; r < NREC; ++r) lds[r * NB + c] = rec[r];
+      PH::load(P, A.U, g.N, idx, fx, fy, lds, NB, c);
     }
   }
 
   // 2. face fluxes, one face per lane per round, kept in registers
-  __device__ static void faces(const Phys<T>& P, const T* lds, int tid, T Fr[ROUNDS][NV]) {
+  __device__ static void faces(const Phys<TP>& P, const void* lds, int tid, TF Fr[ROUNDS][NV]) {
 #pragma unroll
     for (int rd = 0; rd < ROUNDS; ++rd) {
       int f = rd * 256 + tid;
@@ -140,7 +142,8 @@ struct Tile {
   }
 
   // face fluxes -> LDS (after the records are dead)
-  __device__ static void store(T* lds, int tid, const T Fr[ROUNDS][NV]) {
+  __device__ static void store(void* ldsv, int tid, const TF Fr[ROUNDS][NV]) {
+    TF* lds = (TF*)ldsv;
 #pragma unroll
     for (int rd = 0; rd < ROUNDS; ++rd) {
       const int f = rd * 256 + tid;
@@ -152,13 +155,14 @@ struct Tile {
   }
 
   // 3. residual in the reference's order, SSPRK3 update, CFL speed
-  __device__ static void update(const StageArgs<T>& A, const Phys<T>& P, const GridDev& g,
-                                const T* lds, int tid, int bxi, int byi, double dt, T& smax,
+  __device__ static void update(const StageArgs<TS>& A, const Phys<TP>& P, const GridDev& g,
+                                const void* ldsv, int tid, int bxi, int byi, double dt, TS& smax,
                                 unsigned& nbad) {
+    const TF* lds = (const TF*)ldsv;
     const int x0 = bxi * TX, y0 = byi * TY;
     const int64_t N = g.N;
     const double factor = dt / P.vol_d;
-    const T a = T(A.a), b = T(A.b), fb = T(factor * A.b);
+    const TS a = TS(A.a), b = TS(A.b), fb = TS(factor * A.b);
     for (int c = tid; c < SH::NCELL; c += 256) {
       const int ly = c / TX, lx = c - ly * TX;
       const int gx = x0 + lx, gy = y0 + ly;
@@ -168,15 +172,15 @@ struct Tile {
       const int fyl = SH::NFX + ly * TX + lx;
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
-        T r = -lds[v * NF + fxl];
-        r += lds[v * NF + fxl + 1];
+        TS r = -TS(lds[v * NF + fxl]);
+        r += TS(lds[v * NF + fxl + 1]);
         if (DIMS >= 2) {
-          r -= lds[v * NF + fyl];
-          r += lds[v * NF + fyl + TX];
+          r -= TS(lds[v * NF + fyl]);
+          r += TS(lds[v * NF + fyl + TX]);
         }
-        const T u = A.U[v * N + idx];
-        const T u0 = A.U0[v * N + idx];
-        T un = u * b;
+        const TS u = A.U[v * N + idx];
+        const TS u0 = A.U0[v * N + idx];
+        TS un = u * b;
         un += (u0 * a - fb * r);
         A.Uout[v * N + idx] = un;
       }
@@ -189,21 +193,23 @@ struct Tile {
 };
 
 #ifndef TECNO_HOST_BUILD
-template <class PH, typename T, int DIMS, int TX, int TY, int G>
-__global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> A, Phys<T> P, GridDev g) {
-  using TL = Tile<PH, T, DIMS, TX, TY, G>;
-  __shared__ T lds[TL::LDS_N];
+template <class PH, int DIMS, int TX, int TY, int G>
+__global__ __launch_bounds__(256) void stage_kernel(StageArgs<typename PH::TS> A,
+                                                    Phys<typename PH::TP> P, GridDev g) {
+  using TL = Tile<PH, DIMS, TX, TY, G>;
+  using TS = typename PH::TS;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[TL::LDS_BYTES];
   __shared__ double red[4];
   if (!A.ctl->active) return;
   const int tid = threadIdx.x;
   TL::load(A, P, g, lds, tid, blockIdx.x, blockIdx.y);
   __syncthreads();
-  T Fr[TL::ROUNDS][TL::NV];
+  typename PH::TF Fr[TL::ROUNDS][TL::NV];
   TL::faces(P, lds, tid, Fr);
   __syncthreads();
   TL::store(lds, tid, Fr);
   __syncthreads();
-  T smax = T(0);
+  TS smax = TS(0);
   unsigned nbad = 0;
   TL::update(A, P, g, lds, tid, blockIdx.x, blockIdx.y, A.ctl->dt, smax, nbad);
   if (A.last) {

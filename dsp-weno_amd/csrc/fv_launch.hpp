@@ -4,6 +4,7 @@
 // kern_fp32.hip), which instantiate Launch<T>.
 #pragma once
 #include "fv_kernels.hpp"
+#include "fv_mixed.hpp"
 
 namespace tecno {
 
@@ -13,6 +14,8 @@ struct LaunchInfo {
   int G;          // halo width (2 or 3)
   int fast_flux;  // FV_* id for the 2D Euler kernel
   int fast_rec;   // RV_* id
+  int state_fp64; // state held in HBM as double (1) or float (0)
+  int mlp_mode;   // MM_* of the mixed-precision DSP-WENO face
   GridDev grid;
   hipStream_t stream;
   Ctl* ctl;
@@ -104,8 +107,8 @@ __global__ void k_con_to_prim(int system, int dims, int64_t N, const T* U, doubl
 }
 
 // max wave speed of a state (first step's dt)
-template <class PH, typename T>
-__global__ void k_speed(Phys<T> P, int64_t N, const T* U, Ctl* c) {
+template <class PH>
+__global__ void k_speed(Phys<typename PH::TP> P, int64_t N, const typename PH::TS* U, Ctl* c) {
   __shared__ double red[4];
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -229,19 +232,37 @@ __global__ void k_recon_batch(Phys<T> P, const double* st, int ng, int64_t n, do
   r[i] = b;
 }
 
+// Neural_Network::forward on raw stencils.  FP64: the reference arithmetic.
+// FP32: jump form -- cell differences in FP64, features, network, softmax and
+// vertex weights in FP32, the reconstructed values assembled in FP64 from
+// l = u1 + (1-w1) D1/2 + w1 D0/2 and r = u2 - (1-w~0) D1/2 - w~0 D2/2.
 template <typename T>
 __global__ void k_mlp_batch(Phys<T> P, const double* u4, int64_t n, double* l, double* r) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  T a, b;
-  nn_forward(P.net, P.arch5 != 0, T(u4[4 * i]), T(u4[4 * i + 1]), T(u4[4 * i + 2]),
-             T(u4[4 * i + 3]), a, b);
-  l[i] = a;
-  r[i] = b;
+  if constexpr (sizeof(T) == 4) {
+    const double u0 = u4[4 * i], u1 = u4[4 * i + 1], u2 = u4[4 * i + 2], u3 = u4[4 * i + 3];
+    const float d0 = (float)(u1 - u0), d1 = (float)(u2 - u1), d2 = (float)(u3 - u2);
+    const float mx = (float)fmax(fmax(fabs(u0), fabs(u1)), fmax(fabs(u2), fabs(u3)));
+    float st[5], xv[5], yv[5], C1, C2;
+    fast::features(d0, d1, d2, mx, st, xv, yv);
+    fast::body5(P.net, st);
+    fast::head_c(st, xv, yv, C1, C2);
+    const double w1 = 0.25 - 2.0 * (double)C1, tw0 = 0.25 - 2.0 * (double)C2;
+    l[i] = u1 + 0.5 * (1.0 - w1) * (u2 - u1) + 0.5 * w1 * (u1 - u0);
+    r[i] = u2 - 0.5 * (1.0 - tw0) * (u2 - u1) - 0.5 * tw0 * (u3 - u2);
+  } else {
+    T a, b;
+    nn_forward(P.net, P.arch5 != 0, T(u4[4 * i]), T(u4[4 * i + 1]), T(u4[4 * i + 2]),
+               T(u4[4 * i + 3]), a, b);
+    l[i] = a;
+    r[i] = b;
+  }
 }
 
 // FV::num_flux on independent faces.  Scalar: P = n x 2ng.  2D Euler: P, V
-// = n x 2ng x 5 (w and Vz must be zero), F = n x 5.
+// = n x 2ng x 5 (w and Vz must be zero), F = n x 5.  The float instantiation
+// evaluates the mixed-precision face (FP32 flux, FP64 entropy variables).
 template <typename T, int G>
 __global__ void k_flux_batch(Phys<T> P, int system, const double* Pin, const double* Vin,
                              int ng, int64_t n, int ax, double* F) {
@@ -256,23 +277,43 @@ __global__ void k_flux_batch(Phys<T> P, int system, const double* Pin, const dou
     F[i] = f[0];
     return;
   }
-  T loc[E_NREC * S];
-  for (int k = 0; k < S; ++k) {
-    const double* q = Pin + (i * S + k) * 5;
-    const double* v = Vin + (i * S + k) * 5;
-    T rho = q[0], p = q[4];
-    loc[E_RHO * S + k] = rho;
-    loc[E_UX * S + k] = T(q[1]);
-    loc[E_UY * S + k] = T(q[2]);
-    loc[E_P * S + k] = p;
-    loc[E_BK * S + k] = T(0.5) / (P.R * (p / (P.R * rho)));
-    loc[E_VF * S + k] = T(v[0]);
-    loc[E_VX * S + k] = T(v[1]);
-    loc[E_VY * S + k] = T(v[2]);
-    loc[E_VL * S + k] = T(v[4]);
+  double f[4];
+  if constexpr (sizeof(T) == 4 && G == 2) {
+    using PH = Euler2Mixed<RV_GENERIC, MM_SIMT>;
+    double ld[R_N * S];
+    for (int k = 0; k < S; ++k) {
+      const double* q = Pin + (i * S + k) * 5;
+      const double* v = Vin + (i * S + k) * 5;
+      const double beta = 0.5 * q[0] / q[4];
+      ld[R_RHO * S + k] = q[0];
+      ld[R_UX * S + k] = q[1];
+      ld[R_UY * S + k] = q[2];
+      ld[R_BETA * S + k] = beta;
+      ld[R_LRHO * S + k] = log(q[0]);
+      ld[R_LBETA * S + k] = log(beta);
+      ld[R_VF * S + k] = v[0];
+    }
+    PH::face(P, ld, S, 0, 1, ax, f);
+  } else {
+    T loc[E_NREC * S];
+    for (int k = 0; k < S; ++k) {
+      const double* q = Pin + (i * S + k) * 5;
+      const double* v = Vin + (i * S + k) * 5;
+      T rho = q[0], p = q[4];
+      loc[E_RHO * S + k] = rho;
+      loc[E_UX * S + k] = T(q[1]);
+      loc[E_UY * S + k] = T(q[2]);
+      loc[E_P * S + k] = p;
+      loc[E_BK * S + k] = T(0.5) / (P.R * (p / (P.R * rho)));
+      loc[E_VF * S + k] = T(v[0]);
+      loc[E_VX * S + k] = T(v[1]);
+      loc[E_VY * S + k] = T(v[2]);
+      loc[E_VL * S + k] = T(v[4]);
+    }
+    T ft[4];
+    Euler2Ph<T, G, FV_GENERIC, RV_GENERIC>::face(P, loc, S, 0, 1, ax, ft);
+    for (int v = 0; v < 4; ++v) f[v] = ft[v];
   }
-  T f[4];
-  Euler2Ph<T, G, FV_GENERIC, RV_GENERIC>::face(P, loc, S, 0, 1, ax, f);
   F[5 * i + 0] = f[0];
   F[5 * i + 1] = f[1];
   F[5 * i + 2] = f[2];
@@ -283,14 +324,9 @@ __global__ void k_flux_batch(Phys<T> P, int system, const double* Pin, const dou
 // ---------------------------------------------------------------- launchers
 template <typename T>
 struct Launch {
-  static hipError_t stage(const LaunchInfo& L, const Phys<T>& P, const T* U, const T* U0, T* Uout,
-                          double a, double b, int last);
-  static hipError_t speed(const LaunchInfo& L, const Phys<T>& P, const T* U);
-  static hipError_t prim_to_con(const LaunchInfo& L, const double* prim, T* U, double gamma);
-  static hipError_t con_to_prim(const LaunchInfo& L, const T* U, double* prim, double gamma);
-  static hipError_t audit(const LaunchInfo& L, const T* U, double gamma, double* part, int nblk);
-  static hipError_t l1(const LaunchInfo& L, const T* U, const double* exact, double gamma,
-                       double* part, int nblk);
+  static hipError_t stage(const LaunchInfo& L, const Phys<T>& P, const void* U, const void* U0,
+                          void* Uout, double a, double b, int last);
+  static hipError_t speed(const LaunchInfo& L, const Phys<T>& P, const void* U);
   static hipError_t recon_batch(const LaunchInfo& L, const Phys<T>& P, const double* st, int ng,
                                 int64_t n, double* l, double* r);
   static hipError_t mlp_batch(const LaunchInfo& L, const Phys<T>& P, const double* u4, int64_t n,
@@ -299,87 +335,105 @@ struct Launch {
                                const double* Vin, int ng, int64_t n, int ax, double* F);
 };
 
+// kernels on the state, instantiated for the state type
+template <typename TS>
+struct StateOps {
+  static hipError_t prim_to_con(const LaunchInfo& L, const double* prim, void* U, double gamma) {
+    int nblk = (int)((L.grid.N + 255) / 256);
+    hipLaunchKernelGGL((k_prim_to_con<TS>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims,
+                       L.grid.N, prim, (TS*)U, gamma);
+    return hipGetLastError();
+  }
+  static hipError_t con_to_prim(const LaunchInfo& L, const void* U, double* prim, double gamma) {
+    int nblk = (int)((L.grid.N + 255) / 256);
+    hipLaunchKernelGGL((k_con_to_prim<TS>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims,
+                       L.grid.N, (const TS*)U, prim, gamma);
+    return hipGetLastError();
+  }
+  static hipError_t audit(const LaunchInfo& L, const void* U, double gamma, double* part,
+                          int nblk) {
+    hipLaunchKernelGGL((k_audit<TS>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims,
+                       L.grid.N, (const TS*)U, gamma, part);
+    return hipGetLastError();
+  }
+  static hipError_t l1(const LaunchInfo& L, const void* U, const double* exact, double gamma,
+                       double* part, int nblk) {
+    hipLaunchKernelGGL((k_l1<TS>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims, L.grid.N,
+                       (const TS*)U, exact, gamma, part);
+    return hipGetLastError();
+  }
+};
+
 #define TECNO_TILE2_X 32
 #define TECNO_TILE2_Y 15
 #define TECNO_TILE1_X 255
 
-template <class PH, typename T, int DIMS, int TX, int TY, int G>
-static hipError_t launch_stage_t(const LaunchInfo& L, const Phys<T>& P, const StageArgs<T>& A) {
+template <class PH, int DIMS, int TX, int TY, int G>
+static hipError_t launch_stage_t(const LaunchInfo& L, const Phys<typename PH::TP>& P,
+                                 const void* U, const void* U0, void* Uo, double a, double b,
+                                 int last) {
+  using TS = typename PH::TS;
+  StageArgs<TS> A{(const TS*)U, (const TS*)U0, (TS*)Uo, L.ctl, a, b, last};
   dim3 grid((L.grid.n[0] + TX - 1) / TX, DIMS >= 2 ? (L.grid.n[1] + TY - 1) / TY : 1, 1);
-  hipLaunchKernelGGL((stage_kernel<PH, T, DIMS, TX, TY, G>), grid, dim3(256), 0, L.stream, A, P,
+  hipLaunchKernelGGL((stage_kernel<PH, DIMS, TX, TY, G>), grid, dim3(256), 0, L.stream, A, P,
                      L.grid);
   return hipGetLastError();
 }
 
+template <class PH>
+static hipError_t launch_speed_t(const LaunchInfo& L, const Phys<typename PH::TP>& P,
+                                 const void* U) {
+  int nblk = (int)((L.grid.N + 255) / 256);
+  if (nblk > 4096) nblk = 4096;
+  hipLaunchKernelGGL((k_speed<PH>), dim3(nblk), dim3(256), 0, L.stream, P, L.grid.N,
+                     (const typename PH::TS*)U, L.ctl);
+  return hipGetLastError();
+}
+
 template <typename T>
-hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const T* U, const T* U0,
-                            T* Uout, double a, double b, int last) {
-  StageArgs<T> A{U, U0, Uout, L.ctl, a, b, last};
+hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U, const void* U0,
+                            void* Uo, double a, double b, int last) {
   constexpr int X2 = TECNO_TILE2_X, Y2 = TECNO_TILE2_Y, X1 = TECNO_TILE1_X;
   if (L.system == 0) {
     if (L.dims == 1)
-      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, T, 1, X1, 1, 3>(L, P, A)
-                      : launch_stage_t<ScalarPh<T, 2>, T, 1, X1, 1, 2>(L, P, A);
+      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 1, X1, 1, 3>(L, P, U, U0, Uo, a, b, last)
+                      : launch_stage_t<ScalarPh<T, 2>, 1, X1, 1, 2>(L, P, U, U0, Uo, a, b, last);
     if (L.dims == 2)
-      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, T, 2, X2, Y2, 3>(L, P, A)
-                      : launch_stage_t<ScalarPh<T, 2>, T, 2, X2, Y2, 2>(L, P, A);
+      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 2, X2, Y2, 3>(L, P, U, U0, Uo, a, b, last)
+                      : launch_stage_t<ScalarPh<T, 2>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
     return hipErrorNotSupported;
   }
-  if (L.dims == 2) {
+  if (L.dims != 2) return hipErrorNotSupported;
+  if constexpr (sizeof(T) == 4) {
+    // mixed precision: kepes_tecno4_roe with sp_weno / sp_wenoc or sp_weno_dl
+    if (L.fast_flux != FV_KEPES4_ROE || L.G != 2 || L.fast_rec == RV_GENERIC)
+      return hipErrorNotSupported;
+    if (L.fast_rec == RV_SPWENO)
+      return launch_stage_t<Euler2Mixed<RV_SPWENO, MM_SIMT>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b,
+                                                                          last);
+    return launch_stage_t<Euler2Mixed<RV_DSP, MM_SIMT>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
+  } else {
     if (L.G == 3)
-      return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_GENERIC>, T, 2, X2, Y2, 3>(L, P, A);
+      return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 3>(L, P, U, U0, Uo,
+                                                                                 a, b, last);
     if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_SPWENO)
-      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_SPWENO>, T, 2, X2, Y2, 2>(L, P, A);
+      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_SPWENO>, 2, X2, Y2, 2>(L, P, U, U0,
+                                                                                    Uo, a, b, last);
     if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_DSP)
-      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_DSP>, T, 2, X2, Y2, 2>(L, P, A);
-    return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, T, 2, X2, Y2, 2>(L, P, A);
+      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_DSP>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
+                                                                                 a, b, last);
+    return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
+                                                                               a, b, last);
   }
-  return hipErrorNotSupported;
 }
 
 template <typename T>
-hipError_t Launch<T>::speed(const LaunchInfo& L, const Phys<T>& P, const T* U) {
-  int nblk = (int)((L.grid.N + 255) / 256);
-  if (nblk > 4096) nblk = 4096;
-  if (L.system == 0)
-    hipLaunchKernelGGL((k_speed<ScalarPh<T, 2>, T>), dim3(nblk), dim3(256), 0, L.stream, P,
-                       L.grid.N, U, L.ctl);
+hipError_t Launch<T>::speed(const LaunchInfo& L, const Phys<T>& P, const void* U) {
+  if (L.system == 0) return launch_speed_t<ScalarPh<T, 2>>(L, P, U);
+  if constexpr (sizeof(T) == 4)
+    return launch_speed_t<Euler2Mixed<RV_SPWENO, MM_SIMT>>(L, P, U);
   else
-    hipLaunchKernelGGL((k_speed<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, T>), dim3(nblk), dim3(256),
-                       0, L.stream, P, L.grid.N, U, L.ctl);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t Launch<T>::prim_to_con(const LaunchInfo& L, const double* prim, T* U, double gamma) {
-  int nblk = (int)((L.grid.N + 255) / 256);
-  hipLaunchKernelGGL((k_prim_to_con<T>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims,
-                     L.grid.N, prim, U, gamma);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t Launch<T>::con_to_prim(const LaunchInfo& L, const T* U, double* prim, double gamma) {
-  int nblk = (int)((L.grid.N + 255) / 256);
-  hipLaunchKernelGGL((k_con_to_prim<T>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims,
-                     L.grid.N, U, prim, gamma);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t Launch<T>::audit(const LaunchInfo& L, const T* U, double gamma, double* part,
-                            int nblk) {
-  hipLaunchKernelGGL((k_audit<T>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims, L.grid.N,
-                     U, gamma, part);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t Launch<T>::l1(const LaunchInfo& L, const T* U, const double* exact, double gamma,
-                         double* part, int nblk) {
-  hipLaunchKernelGGL((k_l1<T>), dim3(nblk), dim3(256), 0, L.stream, L.system, L.dims, L.grid.N, U,
-                     exact, gamma, part);
-  return hipGetLastError();
+    return launch_speed_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>>(L, P, U);
 }
 
 template <typename T>

@@ -1,0 +1,454 @@
+// fv_mixed.hpp — the mixed-precision 2D Euler TeCNO face (the FP32 path).
+//
+// State in HBM, per-cell records and the entropy-conservative flux are FP64
+// (the residual is a difference of neighbouring fluxes, so their rounding
+// error enters it divided by h); the entropy dissipation -- eigensystem,
+// reconstruction of the scaled entropy variables, DSP-WENO network -- is
+// about 1e-5 of the flux in smooth flow and runs in FP32 with the hardware
+// reciprocal / log2 / exp2 / sqrt.
+// Reconstruction works on jumps: with D0 = u1-u0, D1 = u2-u1, D2 = u3-u2 the
+// reconstructed jump of SP-WENO (limiter.h:193-252) and of DSP-WENO
+// (neural_network.cc:16-77, 320-325) is, exactly in real arithmetic,
+//     u+ - u- = 0.5 * ( w1 (D1 - D0) + w~0 (D1 - D2) ),
+// w1 = 1/4 - 2 C1, w~0 = 1/4 - 2 C2, so only differences of the scaled
+// entropy variables enter.  The differences are taken in FP64 (V is FP64 per
+// cell) and projected with the FP32 eigenvectors, so smooth-flow jumps keep
+// FP32 relative accuracy instead of losing it to cancellation.
+#pragma once
+#include "fv_physics.hpp"
+
+namespace tecno {
+namespace fast {
+
+#ifdef TECNO_HOST_BUILD
+__device__ __forceinline__ float rcp(float x) { return 1.0f / x; }
+__device__ __forceinline__ float log(float x) { return std::log(x); }
+__device__ __forceinline__ float exp(float x) { return std::exp(x); }
+__device__ __forceinline__ float sqrt(float x) { return std::sqrt(x); }
+#else
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float log(float x) {
+  return __builtin_amdgcn_logf(x) * 0.69314718055994531f;  // v_log_f32 is log2
+}
+__device__ __forceinline__ float exp(float x) {
+  return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+}
+__device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+#endif
+__device__ __forceinline__ float div(float a, float b) { return a * rcp(b); }
+__device__ __forceinline__ float tanh(float x) {
+  // 1 - 2/(e^{2x}+1): saturates to +-1 for large |x|; absolute error ~1e-7
+  return 1.0f - 2.0f * rcp(exp(2.0f * x) + 1.0f);
+}
+
+// logarithmic mean (misc_func.h:14-32); both branches are well conditioned in
+// FP32 for positive arguments
+__device__ __forceinline__ float logavg(float a, float b) {
+  const float xi = b * rcp(a);
+  const float f = (xi - 1.0f) * rcp(xi + 1.0f);
+  const float u = f * f;
+  float F;
+  if (u < 1.0e-2f)
+    F = 1.0f + u * (1.0f / 3.0f) + (u * u) * (1.0f / 5.0f) + (u * u * u) * (1.0f / 7.0f);
+  else
+    F = log(xi) * rcp(2.0f * f);
+  return 0.5f * (a + b) * rcp(F);
+}
+
+// ---------------------------------------------------------------- SP-WENO
+__device__ __forceinline__ float spw_fp(float t0, float t1) {
+  const float eps = 1.0e-13f;
+  float psi = (fabsf(t0 - 1.0f) < eps) ? 0.0f : (1.0f - t1) * rcp(1.0f - t0);
+  return (fabsf(t0 - 1.0f) > eps && fabsf(psi + 1.0f) > eps) ? rcp(1.0f + psi) : 1.0f;
+}
+
+__device__ __forceinline__ float spw_cp(float t0, float t1, float del1, float scale, bool wc,
+                                        float& corr) {
+  const float eps = 1.0e-13f;
+  float psi;
+  if (fabsf(t0 - 1.0f) < eps || fabsf(t1 - 1.0f) < eps)
+    psi = 0.0f;
+  else
+    psi = (1.0f - t1) * rcp(1.0f - t0);
+  float C = 0.0f;
+  corr = 0.0f;
+  if (fabsf(psi + 1.0f) > eps && fabsf(t0 - 1.0f) > eps && psi < 0.0f) {
+    const float a = spw_fp(t0, t1), b = spw_fp(t1, t0);
+    C = a * rcp(a * a + b * b) * 0.125f;
+    if (wc) {
+      const float m = fminf(fabsf(del1 * rcp(scale)), fabsf(del1));
+      corr = -0.25f * m * m * m * rcp(1.0f - t0);
+    }
+  } else if (fabsf(t0 - 1.0f) > eps && fabsf(psi + 1.0f) < eps) {
+    C = 0.0f;
+  } else if (fabsf(t0 - 1.0f) < eps || (psi > -eps && fabsf(t0) < 1.0f + eps)) {
+    C = -0.375f;
+  } else if (psi > -eps && fabsf(t0) > 1.0f) {
+    C = 0.125f;
+  }
+  return C;
+}
+
+// reconstructed jump of SP-WENO from the cell jumps (u1, u2 only for sp_wenoc)
+__device__ __forceinline__ float spw_jump(float d0, float d1, float d2, float u1, float u2,
+                                          int corr) {
+  if (fabsf(d1) < 1.0e-13f) return d1;  // limiter.h:202-206: (u1, u2)
+  const float r = rcp(d1);
+  const float tp = d0 * r, tm = d2 * r;
+  const bool wc = corr == 1;
+  const float scale = 0.5f * (fabsf(u1) + fabsf(u2));
+  float c1c, c2c;
+  float C1 = spw_cp(tp, tm, d1, scale, wc, c1c);
+  float C2 = spw_cp(tm, tp, d1, scale, wc, c2c);
+  if (wc) {
+    C1 = fminf(fmaxf(C1 + c1c, -0.375f), 0.125f);
+    C2 = fminf(fmaxf(C2 + c2c, -0.375f), 0.125f);
+  }
+  return 0.5f * ((0.25f - 2.0f * C1) * (d1 - d0) + (0.25f - 2.0f * C2) * (d1 - d2));
+}
+
+// ---------------------------------------------------------------- DSP-WENO
+// compute_vertices (neural_network.cc:135-260, vertex_mode 2, permutation 2)
+__device__ __forceinline__ void vertices(float tm, float tp, float maxd, float xv[5], float yv[5]) {
+  const float thr = 1e-8f;
+  const float psi_p = (1.0f - tm) * rcp(1.0f - tp);
+  const float psi_m = rcp(psi_p);
+  const float g1 = fminf(maxd, 0.125f);
+  const float g2 = -fminf(maxd, 0.375f);
+  const float x1 = 0.125f * (1.0f + psi_p) - g1 * psi_p;
+  const float y1 = 0.125f * (1.0f + psi_m) - g2 * psi_m;
+  const float x2 = 0.125f * (1.0f + psi_p) - g2 * psi_p;
+  const float y2 = 0.125f * (1.0f + psi_m) - g1 * psi_m;
+  const float e = 0.125f, t = -0.375f, m = -0.125f;
+  const float nx = ((psi_m * psi_m) + psi_m) * rcp(8.0f * ((psi_m * psi_m) + 1.0f));
+  const float ny = 0.125f * (1.0f + psi_m) - nx * psi_m;
+#define FV_SET(a0, a1, a2, a3, a4, b0, b1, b2, b3, b4) \
+  { xv[0] = a0; xv[1] = a1; xv[2] = a2; xv[3] = a3; xv[4] = a4; \
+    yv[0] = b0; yv[1] = b1; yv[2] = b2; yv[3] = b3; yv[4] = b4; }
+  if ((tm - 1.0f) > thr && (tp - 1.0f) > thr) {
+    FV_SET(e, e, e, e, e, e, e, e, e, e)
+  } else if ((tm - 1.0f) > thr && (tp - 1.0f) < -thr) {
+    if (psi_p < -1.0f) {
+      if (x1 < g2) FV_SET(nx, nx, nx, nx, nx, ny, ny, ny, ny, ny)
+      else {
+        const float cx = (2.0f * g2 + x1) * (1.0f / 3.0f), cy = (2.0f * g1 + y2) * (1.0f / 3.0f);
+        FV_SET(g2, x1, g2, cx, cx, g1, g1, y2, cy, cy)
+      }
+    } else {
+      if (y2 < g2) FV_SET(g2, g1, g2, g1, 0.0f, g1, g1, g2, g2, 0.0f)
+      else FV_SET(g2, g1, g2, g1, x2, g1, g1, g2, y2, g2)
+    }
+  } else if ((tm - 1.0f) < -thr && (tp - 1.0f) > thr) {
+    if (psi_p < -1.0f) {
+      if (x1 < g2) FV_SET(g2, g1, g2, g1, 0.0f, g1, g1, g2, g2, 0.0f)
+      else FV_SET(g1, g1, g2, x1, g2, g2, g1, g2, g1, y1)
+    } else {
+      if (y2 < g2) FV_SET(nx, nx, nx, nx, nx, ny, ny, ny, ny, ny)
+      else {
+        const float cx = (2.0f * g1 + x1) * (1.0f / 3.0f), cy = (2.0f * g2 + y2) * (1.0f / 3.0f);
+        FV_SET(g1, x1, g1, cx, cx, g2, g2, y2, cy, cy)
+      }
+    }
+  } else if (fabsf(tm - 1.0f) < thr && (tp - 1.0f) > thr) {
+    FV_SET(e, e, e, e, e, t, e, e, t, m)
+  } else if (fabsf(tp - 1.0f) < thr && (tm - 1.0f) > thr) {
+    FV_SET(e, e, t, t, m, e, e, e, e, e)
+  } else {
+    FV_SET(e, e, t, t, m, e, t, t, e, m)
+  }
+#undef FV_SET
+}
+
+// features of one component from its jumps and the stencil magnitude
+__device__ __forceinline__ void features(float d0, float d1, float d2, float maxabs, float st[5],
+                                         float xv[5], float yv[5]) {
+  const float inv = rcp(fmaxf(1.0f, maxabs));
+  const float r = rcp(d1);
+  const float tm = d2 * r, tp = d0 * r;
+  const float a1 = fabsf(d0) * inv, a2 = fabsf(d1) * inv, a3 = fabsf(d2) * inv;
+  st[0] = tanh(tm);
+  st[1] = tanh(tp);
+  st[2] = a1;
+  st[3] = a2;
+  st[4] = a3;
+  vertices(tm, tp, fmaxf(fmaxf(a1, a2), a3), xv, yv);
+}
+
+// softmax with zero-initialised maximum + vertex weights -> reconstructed jump
+__device__ __forceinline__ float head(const float lg[5], const float xv[5], const float yv[5],
+                                      float d0, float d1, float d2) {
+  float mx = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) mx = fmaxf(mx, lg[i]);
+  float s[5], sum = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    s[i] = exp(lg[i] - mx);
+    sum += s[i];
+  }
+  const float inv = rcp(sum);
+  float C1 = 0.0f, C2 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    C1 += s[i] * xv[i];
+    C2 += s[i] * yv[i];
+  }
+  C1 *= inv;
+  C2 *= inv;
+  return 0.5f * ((0.25f - 2.0f * C1) * (d1 - d0) + (0.25f - 2.0f * C2) * (d1 - d2));
+}
+
+// softmax + vertex combination -> (C1, C2)
+__device__ __forceinline__ void head_c(const float lg[5], const float xv[5], const float yv[5],
+                                       float& C1, float& C2) {
+  float mx = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) mx = fmaxf(mx, lg[i]);
+  float s[5], sum = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    s[i] = exp(lg[i] - mx);
+    sum += s[i];
+  }
+  const float inv = rcp(sum);
+  C1 = 0.0f;
+  C2 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    C1 += s[i] * xv[i];
+    C2 += s[i] * yv[i];
+  }
+  C1 *= inv;
+  C2 *= inv;
+}
+
+// 5-5-5-5-5 network body, FP32 SIMT
+__device__ __forceinline__ void body5(const NetView<float>& net, float st[5]) {
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const float* W = net.W + l * TECNO_NET_MAXW * TECNO_NET_MAXW;
+    const float* b = net.b + l * TECNO_NET_MAXW;
+    float o[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      float dot = b[j];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) dot = fmaf(W[j * TECNO_NET_MAXW + k], st[k], dot);
+      o[j] = dot;
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) st[j] = (l != 3) ? fmaxf(0.0f, o[j]) : o[j];
+  }
+}
+
+}  // namespace fast
+
+// ---------------------------------------------------------------- FP64 helpers
+#ifdef TECNO_HOST_BUILD
+__device__ __forceinline__ double rcp64(double x) { return 1.0 / x; }
+#else
+// v_rcp_f64 refined by two Newton steps (about 1 ulp)
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+}
+#endif
+
+// logarithmic mean with the logarithms of the arguments known per cell:
+// f = (b-a)/(b+a) = (xi-1)/(xi+1), ln xi = ln b - ln a (misc_func.h:14-32)
+__device__ __forceinline__ double logavg_ln(double a, double b, double la, double lb) {
+  const double f = (b - a) * rcp64(b + a);
+  const double u = f * f;
+  double F;
+  if (u < 1.0e-2)
+    F = 1.0 + u * (1.0 / 3.0) + (u * u) * (1.0 / 5.0) + (u * u * u) * (1.0 / 7.0);
+  else
+    F = (lb - la) * rcp64(2.0 * f);
+  return 0.5 * (a + b) * rcp64(F);
+}
+
+// ---------------------------------------------------------------- the face
+enum { MM_SIMT = 0, MM_BF16 = 1, MM_FP16 = 2 };
+
+// record layout in LDS, double [7][NB]: rho, ux, uy, beta = rho/(2p),
+// ln rho, ln beta, Vf (first entropy variable)
+enum { R_RHO = 0, R_UX, R_UY, R_BETA, R_LRHO, R_LBETA, R_VF, R_N };
+
+template <int RV, int MM>
+struct Euler2Mixed {
+  using TS = double;
+  using TF = double;
+  using TP = float;
+  static constexpr int G = 2;
+  static constexpr int NV = 4;
+  static constexpr size_t rec_bytes(int NB) { return (size_t)R_N * NB * 8; }
+
+  __device__ static void load(const Phys<float>& P, const double* U, int64_t N, int64_t idx,
+                              int flipx, int flipy, void* lds, int NB, int c) {
+    const double rho = U[idx], mx = U[N + idx], my = U[2 * N + idx], E = U[3 * N + idx];
+    const double inv = rcp64(rho);
+    double ux = mx * inv, uy = my * inv;
+    const double p = P.gm1_d * (E - 0.5 * (mx * mx + my * my) * inv);
+    if (flipx) ux = -ux;
+    if (flipy) uy = -uy;
+    const double beta = 0.5 * rho * rcp64(p);
+    const double lr = log(rho), lp = log(p);
+    const double s = lp - P.gamma_d * lr;  // Entropy (fv.cc:557-560)
+    double* ld = (double*)lds;
+    ld[R_RHO * NB + c] = rho;
+    ld[R_UX * NB + c] = ux;
+    ld[R_UY * NB + c] = uy;
+    ld[R_BETA * NB + c] = beta;
+    ld[R_LRHO * NB + c] = lr;
+    ld[R_LBETA * NB + c] = lr - lp - 0.69314718055994531;  // ln(rho/(2p))
+    ld[R_VF * NB + c] = (P.gamma_d - s) * P.inv_gm1_d - (ux * ux + uy * uy) * beta;
+  }
+
+  // kepec_flux (kepec.cc:10-48) per unit area in the face frame, FP64
+  __device__ static void kepec(const Phys<float>& P, double rl, double nl, double tl, double bl,
+                               double lrl, double lbl, double rr, double nr, double tr, double br,
+                               double lrr, double lbr, double F[4], double* aux = nullptr) {
+    const double rho = logavg_ln(rl, rr, lrl, lrr);
+    const double beta = logavg_ln(bl, br, lbl, lbr);
+    const double vel2 = 0.5 * ((nl * nl + tl * tl) + (nr * nr + tr * tr));
+    const double p = 0.5 * (rl + rr) * rcp64(bl + br);
+    const double vn = 0.5 * (nl + nr), vt = 0.5 * (tl + tr);
+    const double mass = rho * vn;
+    F[0] = mass;
+    F[1] = p + vn * mass;
+    F[2] = vt * mass;
+    F[3] = 0.5 * (rcp64(P.gm1_d * beta) - vel2) * mass + (F[1] * vn + F[2] * vt);
+    if (aux) {
+      aux[0] = rho;
+      aux[1] = beta;
+      aux[2] = p;
+      aux[3] = vn;
+      aux[4] = vt;
+    }
+  }
+
+  __device__ static void face(const Phys<float>& P, const void* ldsv, int NB, int base, int stride,
+                              int ax, double F[4]) {
+    const double* ld = (const double*)ldsv;
+    const int rn = ax ? R_UY : R_UX, rt = ax ? R_UX : R_UY;
+    double rho[4], un[4], ut[4], be[4], lr[4], lb[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int c = base + l * stride;
+      rho[l] = ld[R_RHO * NB + c];
+      un[l] = ld[rn * NB + c];
+      ut[l] = ld[rt * NB + c];
+      be[l] = ld[R_BETA * NB + c];
+      lr[l] = ld[R_LRHO * NB + c];
+      lb[l] = ld[R_LBETA * NB + c];
+    }
+    // ---- entropy-conservative flux, 4th order (kepec4.cc:10-38), FP64
+    double Fe[4], f0[4], aux[5];
+    kepec(P, rho[1], un[1], ut[1], be[1], lr[1], lb[1], rho[2], un[2], ut[2], be[2], lr[2], lb[2],
+          Fe, aux);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fe[v] *= (4.0 / 3.0);
+    kepec(P, rho[0], un[0], ut[0], be[0], lr[0], lb[0], rho[2], un[2], ut[2], be[2], lr[2], lb[2],
+          f0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fe[v] -= (1.0 / 6.0) * f0[v];
+    kepec(P, rho[1], un[1], ut[1], be[1], lr[1], lb[1], rho[3], un[3], ut[3], be[3], lr[3], lb[3],
+          f0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fe[v] -= (1.0 / 6.0) * f0[v];
+
+    // ---- eigensystem at the face (kepes_tecno4_roe.cc:39-113) and the
+    // projection of the entropy-variable jumps, FP64: a jump of a scaled
+    // variable can be a near-total cancellation of its terms (e.g. the shear
+    // wave at a symmetric extremum), and the zero-jump tests (1e-13 / 1e-15)
+    // are absolute
+    const double rhoh = aux[0], betah = aux[1], ph = aux[2], vn = aux[3], vt = aux[4];
+    const double a = sqrt(0.5 * P.gamma_d * rcp64(betah));
+    const double vsq = vn * vn + vt * vt;
+    const double H = a * a * P.inv_gm1_d + 0.5 * vsq;
+    const double c1 = sqrt(rhoh * P.inv_gamma_d);
+    const double S0 = P.s05_d * c1, S1 = P.sgm1_d * c1, S2 = sqrt(ph);
+    const double Rn0 = (vn - a) * S0, Rt0 = vt * S0, RE0 = (H - a * vn) * S0;
+    const double Rn1 = vn * S1, Rt1 = vt * S1, RE1 = (0.5 * vsq) * S1;
+    const double Rt2 = -S2, RE2 = -vt * S2;
+    const double Rn4 = (vn + a) * S0, Rt4 = Rt0, RE4 = (H + a * vn) * S0;
+    double Vf[4], Vn[4], Vt[4], Vl[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      Vf[l] = ld[R_VF * NB + base + l * stride];
+      Vn[l] = 2.0 * be[l] * un[l];
+      Vt[l] = 2.0 * be[l] * ut[l];
+      Vl[l] = -2.0 * be[l];
+    }
+    float dZ[4][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double df = Vf[j + 1] - Vf[j], dn = Vn[j + 1] - Vn[j];
+      const double dt = Vt[j + 1] - Vt[j], dl = Vl[j + 1] - Vl[j];
+      dZ[0][j] = (float)(S0 * df + Rn0 * dn + Rt0 * dt + RE0 * dl);
+      dZ[1][j] = (float)(S1 * df + Rn1 * dn + Rt1 * dt + RE1 * dl);
+      dZ[2][j] = (float)(Rt2 * dt + RE2 * dl);
+      dZ[3][j] = (float)(S0 * df + Rn4 * dn + Rt4 * dt + RE4 * dl);
+    }
+    float Z1[4];
+    Z1[0] = (float)(S0 * Vf[1] + Rn0 * Vn[1] + Rt0 * Vt[1] + RE0 * Vl[1]);
+    Z1[1] = (float)(S1 * Vf[1] + Rn1 * Vn[1] + Rt1 * Vt[1] + RE1 * Vl[1]);
+    Z1[2] = (float)(Rt2 * Vt[1] + RE2 * Vl[1]);
+    Z1[3] = (float)(S0 * Vf[1] + Rn4 * Vn[1] + Rt4 * Vt[1] + RE4 * Vl[1]);
+    float jmp[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float d0 = dZ[i][0], d1 = dZ[i][1], d2 = dZ[i][2];
+      const float z1 = Z1[i], z2 = z1 + d1;
+      if (RV == RV_SPWENO || (RV == RV_GENERIC && P.recon != REC_DSP)) {
+        jmp[i] = fast::spw_jump(d0, d1, d2, z1, z2, P.corr);
+      } else {
+        const float z0 = z1 - d0, z3 = z2 + d2;
+        const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
+        float st[5], xv[5], yv[5];
+        fast::features(d0, d1, d2, maxabs, st, xv, yv);
+        fast::body5(P.net, st);
+        float j = fast::head(st, xv, yv, d0, d1, d2);
+        if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
+        // sign property: fall back to SP-WENO where the network's jump has the
+        // wrong sign
+        if (P.sign_fallback && j * d1 < 0.0f) j = fast::spw_jump(d0, d1, d2, z1, z2, 0);
+        jmp[i] = j;
+      }
+    }
+    // ---- dissipation R~ Lambda dZ (FP32) and the TeCNO flux (FP64)
+    const float vnf = (float)vn, af = (float)a;
+    const float L1 = fabsf(vnf), L0 = fabsf(vnf - af), L4 = fabsf(vnf + af);
+    const float s0 = (float)S0, s1 = (float)S1;
+    const float Dr = s0 * L0 * jmp[0] + s1 * L1 * jmp[1] + s0 * L4 * jmp[3];
+    const float Dn = (float)Rn0 * L0 * jmp[0] + (float)Rn1 * L1 * jmp[1] + (float)Rn4 * L4 * jmp[3];
+    const float Dt = (float)Rt0 * L0 * jmp[0] + (float)Rt1 * L1 * jmp[1] + (float)Rt2 * L1 * jmp[2] +
+                     (float)Rt4 * L4 * jmp[3];
+    const float DE = (float)RE0 * L0 * jmp[0] + (float)RE1 * L1 * jmp[1] + (float)RE2 * L1 * jmp[2] +
+                     (float)RE4 * L4 * jmp[3];
+    const double area = P.area_d[ax];
+    const double Fn = (Fe[1] - 0.5 * (double)Dn) * area;
+    const double Ft = (Fe[2] - 0.5 * (double)Dt) * area;
+    F[0] = (Fe[0] - 0.5 * (double)Dr) * area;
+    F[1] = ax ? Ft : Fn;
+    F[2] = ax ? Fn : Ft;
+    F[3] = (Fe[3] - 0.5 * (double)DE) * area;
+  }
+
+  __device__ static double speed(const Phys<float>& P, const double* U, int64_t N, int64_t idx) {
+    const double rho = U[idx], mx = U[N + idx], my = U[2 * N + idx], E = U[3 * N + idx];
+    const double inv = 1.0 / rho;
+    const double ux = mx * inv, uy = my * inv;
+    const double p = P.gm1_d * (E - 0.5 * (mx * mx + my * my) * inv);
+    return sqrt(ux * ux + uy * uy) + sqrt(P.gamma_d * p * inv);
+  }
+  __device__ static bool bad(const double* U, int64_t N, int64_t idx) {
+    const double rho = U[idx], mx = U[N + idx], my = U[2 * N + idx], E = U[3 * N + idx];
+    const double e = E - 0.5 * (mx * mx + my * my) / rho;
+    return !(rho > 0.0) || !(e > 0.0) || !(e < INFINITY);
+  }
+};
+
+}  // namespace tecno

@@ -10,15 +10,20 @@ namespace tecno {
 // -------- scalar (enflo_scalar): record = u
 template <typename T, int G>
 struct ScalarPh {
-  static constexpr int NV = 1, NREC = 1;
+  using TS = T;  // state in HBM
+  using TF = T;  // face flux
+  using TP = T;  // parameters
+  static constexpr int NV = 1;
+  static constexpr size_t rec_bytes(int NB) { return (size_t)NB * sizeof(T); }
   __device__ static void load(const Phys<T>& P, const T* U, int64_t N, int64_t idx, int flipx,
-                              int flipy, T* rec) {
+                              int flipy, void* lds, int NB, int c) {
     T u = U[idx];
-    rec[0] = (flipx || flipy) ? -u : u;  // scalar wall: ghost = -mirror
+    ((T*)lds)[c] = (flipx || flipy) ? -u : u;  // scalar wall: ghost = -mirror
   }
   // face flux: enflo_scalar num_flux.cc:12-57 and lin_/bur_ fluxes
-  __device__ static void face(const Phys<T>& P, const T* lds, int NB, int base, int stride, int ax,
-                              T F[1]) {
+  __device__ static void face(const Phys<T>& P, const void* ldsv, int NB, int base, int stride,
+                              int ax, T F[1]) {
+    const T* lds = (const T*)ldsv;
     T s[2 * G];
 #pragma unroll
     for (int l = 0; l < 2 * G; ++l) s[l] = lds[base + l * stride];
@@ -168,10 +173,17 @@ enum { RV_SPWENO = 0, RV_DSP = 1, RV_GENERIC = 2 };
 
 template <typename T, int G, int FV, int RV>
 struct Euler2Ph {
+  using TS = T;
+  using TF = T;
+  using TP = T;
   static constexpr int NV = 4, NREC = E_NREC;
+  static constexpr size_t rec_bytes(int NB) { return (size_t)NREC * NB * sizeof(T); }
   __device__ static void load(const Phys<T>& P, const T* U, int64_t N, int64_t idx, int flipx,
-                              int flipy, T* rec) {
+                              int flipy, void* lds, int NB, int c) {
+    T rec[NREC];
     euler2_record(P, U[idx], U[N + idx], U[2 * N + idx], U[3 * N + idx], flipx, flipy, rec);
+#pragma unroll
+    for (int r = 0; r < NREC; ++r) ((T*)lds)[r * NB + c] = rec[r];
   }
 
   // TeCNO face flux (kepes_tecno4_roe.cc:10-176 and variants), evaluated in
@@ -179,8 +191,9 @@ struct Euler2Ph {
   // scaled variable and eigenvector entries both change sign on y faces, so
   // the dissipation is unchanged bit for bit.  Sums over the x and y
   // components keep the reference's x-before-y order via `ax`.
-  __device__ static void face(const Phys<T>& P, const T* lds, int NB, int base, int stride, int ax,
-                              T F[4]) {
+  __device__ static void face(const Phys<T>& P, const void* ldsv, int NB, int base, int stride,
+                              int ax, T F[4]) {
+    const T* lds = (const T*)ldsv;
     constexpr int S = 2 * G;
     const int rn = ax ? E_UY : E_UX, rt = ax ? E_UX : E_UY;
     const int vn_ = ax ? E_VY : E_VX, vt_ = ax ? E_VX : E_VY;

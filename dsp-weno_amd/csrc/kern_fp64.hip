@@ -5,4 +5,5 @@
 
 namespace tecno {
 template struct Launch<double>;
+template struct StateOps<double>;
 }

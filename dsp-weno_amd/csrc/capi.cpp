@@ -66,6 +66,7 @@ struct tecno_ctx {
   double* db64 = nullptr;
   float* dW32 = nullptr;
   float* db32 = nullptr;
+  unsigned short* dWmf = nullptr;  // MFMA-packed weights: [bf16 | fp16][4][64][8]
   Phys<double> P64;
   Phys<float> P32;
   LaunchInfo L;
@@ -159,6 +160,9 @@ static void refresh_phys(tecno_ctx* c) {
   c->P64.net.b = c->db64;
   c->P32.net.W = c->dW32;
   c->P32.net.b = c->db32;
+  c->P32.mfma_w[0] = c->dWmf;
+  c->P32.mfma_w[1] = c->dWmf ? c->dWmf + 4 * 64 * 8 : nullptr;
+  c->P64.mfma_w[0] = c->P64.mfma_w[1] = nullptr;
 }
 
 static int ensure_scratch(tecno_ctx* c, size_t bytes) {
@@ -184,6 +188,36 @@ static int upload_net(tecno_ctx* c) {
   HIPCHK(hipMemcpy(c->db64, c->net.b.data(), nb * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->dW32, w32.data(), nw * sizeof(float), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->db32, b32.data(), nb * sizeof(float), hipMemcpyHostToDevice));
+  // per-lane packed weights for the matrix-core network (5-5-5-5-5 only)
+  if (c->net.nl == 4) {
+    bool all5 = true;
+    for (int i = 0; i <= 4; ++i) all5 = all5 && c->net.w[i] == 5;
+    if (all5) {
+      std::vector<unsigned short> pk(2 * 4 * 64 * 8, 0);
+      for (int l = 0; l < 4; ++l)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int m = lane % 32, half = lane / 32;
+          int j = -1;  // output neuron of row m for this lane's half
+          if (half == 0 && (m < 4 || m == 8)) j = m < 4 ? m : 4;
+          if (half == 1 && ((m >= 4 && m < 8) || m == 12)) j = m < 8 ? m - 4 : 4;
+          for (int e = 0; e < 5; ++e) {
+            const float w = j < 0 ? 0.0f
+                                  : (float)c->net.W[(size_t)l * TECNO_NET_MAXW * TECNO_NET_MAXW +
+                                                    j * TECNO_NET_MAXW + e];
+            uint32_t u;
+            std::memcpy(&u, &w, 4);
+            u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+            pk[((size_t)l * 64 + lane) * 8 + e] = (unsigned short)(u >> 16);
+            _Float16 h = (_Float16)w;
+            unsigned short hb;
+            std::memcpy(&hb, &h, 2);
+            pk[4 * 64 * 8 + ((size_t)l * 64 + lane) * 8 + e] = hb;
+          }
+        }
+      if (!c->dWmf) HIPCHK(hipMalloc(&c->dWmf, pk.size() * sizeof(unsigned short)));
+      HIPCHK(hipMemcpy(c->dWmf, pk.data(), pk.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
+    }
+  }
   refresh_phys(c);
   return 0;
 }
@@ -314,6 +348,9 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   c->L.dims = g.dims;
   c->L.G = c->G;
   c->L.fast_flux = g.flux == TECNO_FLUX_KEPES_TECNO4_ROE && !g.ec1_fix ? FV_KEPES4_ROE : FV_GENERIC;
+  c->L.mlp_mode = g.mlp_dtype == TECNO_MLP_BF16 ? MM_BF16
+                  : g.mlp_dtype == TECNO_MLP_FP16 ? MM_FP16
+                                                   : MM_SIMT;
   c->L.fast_rec = (g.recon == TECNO_REC_SP_WENO) ? RV_SPWENO
                   : (g.recon == TECNO_REC_SP_WENO_DL) ? RV_DSP
                                                       : RV_GENERIC;
@@ -338,6 +375,7 @@ void tecno_destroy(tecno_ctx* c) {
   if (c->db64) (void)hipFree(c->db64);
   if (c->dW32) (void)hipFree(c->dW32);
   if (c->db32) (void)hipFree(c->db32);
+  if (c->dWmf) (void)hipFree(c->dWmf);
   for (auto& e : c->ev) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
@@ -462,6 +500,15 @@ int tecno_num_vars(const tecno_ctx* c) { return c ? c->nprim : 0; }
 static int need_net(tecno_ctx* c) {
   if (c->cfg.recon == TECNO_REC_SP_WENO_DL && !c->has_net)
     return fail(TECNO_ESTATE, "sp_weno_dl needs a network (tecno_load_net)");
+  // every kernel that evaluates the network dereferences these pointers
+  if (c->cfg.recon == TECNO_REC_SP_WENO_DL &&
+      (!c->P64.net.W || !c->P64.net.b || !c->P32.net.W || !c->P32.net.b))
+    return fail(TECNO_ESTATE, "network weights are not on the device");
+  if (c->cfg.recon == TECNO_REC_SP_WENO_DL && c->L.mlp_mode != MM_SIMT &&
+      (!c->P32.mfma_w[0] || !c->P32.mfma_w[1]))
+    return fail(TECNO_ESTATE, "matrix-core weights are not on the device");
+  if (c->cfg.recon == TECNO_REC_SP_WENO_DL && c->L.mlp_mode != MM_SIMT && !c->P32.arch5)
+    return fail(TECNO_EINVAL, "the matrix-core network needs the 5-5-5-5-5 architecture");
   return 0;
 }
 
@@ -723,12 +770,17 @@ int tecno_reconstruct_batch(tecno_ctx* c, const double* st, int64_t n, double* l
 int tecno_mlp_forward_batch(tecno_ctx* c, const double* u4, int64_t n, double* l, double* r) {
   if (!c || !u4 || !l || !r) return fail(TECNO_EINVAL, "null argument");
   if (!c->has_net) return fail(TECNO_ESTATE, "no network loaded");
+  if (!c->P64.net.W || !c->P32.net.W || !c->db32)
+    return fail(TECNO_ESTATE, "network weights are not on the device");
   if (n <= 0) return 0;
+  if ((c->cfg.mlp_dtype == TECNO_MLP_BF16 || c->cfg.mlp_dtype == TECNO_MLP_FP16) && !c->P32.arch5)
+    return fail(TECNO_EINVAL, "the matrix-core network needs the 5-5-5-5-5 architecture");
   if (c->cfg.mlp_dtype == TECNO_MLP_BF16 || c->cfg.mlp_dtype == TECNO_MLP_FP16)
     return with_device_io(c, {{u4, (size_t)n * 4}}, {{l, (size_t)n}, {r, (size_t)n}},
                           [&](std::vector<double*>& i, std::vector<double*>& o) {
-                            return mlp_mfma_batch(c->stream, c->dW32, c->db32, c->net.nl,
-                                                  c->cfg.mlp_dtype == TECNO_MLP_FP16, i[0], n, o[0], o[1]);
+                            return mlp_mfma_batch(c->stream, c->P32.mfma_w[c->cfg.mlp_dtype == TECNO_MLP_FP16 ? 1 : 0],
+                                                  c->db32, c->cfg.mlp_dtype == TECNO_MLP_FP16, i[0], n,
+                                                  o[0], o[1]);
                           });
   return with_device_io(c, {{u4, (size_t)n * 4}}, {{l, (size_t)n}, {r, (size_t)n}},
                         [&](std::vector<double*>& i, std::vector<double*>& o) {

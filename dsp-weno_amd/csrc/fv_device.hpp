@@ -369,6 +369,7 @@ struct Phys {
   double gamma_d, gm1_d, inv_gm1_d, R_d;
   double area_d[3];
   double s05_d, sgm1_d, inv_gamma_d;
+  const unsigned short* mfma_w[2];  // per-lane packed weights: [0] bf16, [1] fp16
   double vol_d;       // dx*dy*dz (param.in spacings)
   double hmin_d, cfl_d;
   double vel_norm;    // |vel| (dt of linear advection)

@@ -118,7 +118,8 @@ struct Tile {
   }
 
   // 2. face fluxes, one face per lane per round, kept in registers
-  __device__ static void faces(const Phys<TP>& P, const void* lds, int tid, TF Fr[ROUNDS][NV]) {
+  __device__ static void faces(const Phys<TP>& P, const void* lds, int tid, TF Fr[ROUNDS][NV],
+                               unsigned& nfb) {
 #pragma unroll
     for (int rd = 0; rd < ROUNDS; ++rd) {
       int f = rd * 256 + tid;
@@ -137,7 +138,9 @@ struct Tile {
         stride = SH::BX;
         ax = 1;
       }
-      PH::face(P, lds, NB, base, stride, ax, Fr[rd]);
+      unsigned c = 0;
+      PH::face(P, lds, NB, base, stride, ax, Fr[rd], c);
+      if (rd * 256 + tid < NF) nfb += c;
     }
   }
 
@@ -205,7 +208,9 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<typename PH::TS> A
   TL::load(A, P, g, lds, tid, blockIdx.x, blockIdx.y);
   __syncthreads();
   typename PH::TF Fr[TL::ROUNDS][TL::NV];
-  TL::faces(P, lds, tid, Fr);
+  unsigned nfb = 0;
+  TL::faces(P, lds, tid, Fr, nfb);
+  if (nfb) atomicAdd(&A.ctl->nfallback, (unsigned long long)nfb);
   __syncthreads();
   TL::store(lds, tid, Fr);
   __syncthreads();

@@ -273,7 +273,8 @@ __global__ void k_flux_batch(Phys<T> P, int system, const double* Pin, const dou
     T loc[S];
     for (int k = 0; k < S; ++k) loc[k] = T(Pin[i * 2 * ng + k]);
     T f[1];
-    ScalarPh<T, G>::face(P, loc, S, 0, 1, ax, f);
+    unsigned nfb = 0;
+    ScalarPh<T, G>::face(P, loc, S, 0, 1, ax, f, nfb);
     F[i] = f[0];
     return;
   }
@@ -293,7 +294,8 @@ __global__ void k_flux_batch(Phys<T> P, int system, const double* Pin, const dou
       ld[R_LBETA * S + k] = log(beta);
       ld[R_VF * S + k] = v[0];
     }
-    PH::face(P, ld, S, 0, 1, ax, f);
+    unsigned nfb = 0;
+    PH::face(P, ld, S, 0, 1, ax, f, nfb);
   } else {
     T loc[E_NREC * S];
     for (int k = 0; k < S; ++k) {
@@ -311,7 +313,8 @@ __global__ void k_flux_batch(Phys<T> P, int system, const double* Pin, const dou
       loc[E_VL * S + k] = T(v[4]);
     }
     T ft[4];
-    Euler2Ph<T, G, FV_GENERIC, RV_GENERIC>::face(P, loc, S, 0, 1, ax, ft);
+    unsigned nfb = 0;
+    Euler2Ph<T, G, FV_GENERIC, RV_GENERIC>::face(P, loc, S, 0, 1, ax, ft, nfb);
     for (int v = 0; v < 4; ++v) f[v] = ft[v];
   }
   F[5 * i + 0] = f[0];
@@ -411,6 +414,10 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
     if (L.fast_rec == RV_SPWENO)
       return launch_stage_t<Euler2Mixed<RV_SPWENO, MM_SIMT>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b,
                                                                           last);
+    if (L.mlp_mode == MM_BF16)
+      return launch_stage_t<Euler2Mixed<RV_DSP, MM_BF16>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
+    if (L.mlp_mode == MM_FP16)
+      return launch_stage_t<Euler2Mixed<RV_DSP, MM_FP16>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
     return launch_stage_t<Euler2Mixed<RV_DSP, MM_SIMT>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
   } else {
     if (L.G == 3)

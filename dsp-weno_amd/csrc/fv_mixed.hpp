@@ -16,6 +16,7 @@
 // FP32 relative accuracy instead of losing it to cancellation.
 #pragma once
 #include "fv_physics.hpp"
+#include "mlp_mfma.hpp"
 
 namespace tecno {
 namespace fast {
@@ -36,6 +37,17 @@ __device__ __forceinline__ float exp(float x) {
 __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 #endif
 __device__ __forceinline__ float div(float a, float b) { return a * rcp(b); }
+// a / b from the one-ulp reciprocal plus one residual correction (Markstein):
+// exact whenever the quotient is representable, so cell-jump ratios such as
+// theta = Delta0 / Delta1 come out exactly 1 (or -1) for equal (opposite)
+// jumps, as the FP64 region tests of the limiters expect.  b == 0 keeps the
+// uncorrected a * inf.
+__device__ __forceinline__ float quot(float a, float b) {
+  const float r = rcp(b);
+  const float q = a * r;
+  const float qc = fmaf(fmaf(-q, b, a), r, q);
+  return (fabsf(qc) <= 3.402823466e38f) ? qc : q;
+}
 __device__ __forceinline__ float tanh(float x) {
   // 1 - 2/(e^{2x}+1): saturates to +-1 for large |x|; absolute error ~1e-7
   return 1.0f - 2.0f * rcp(exp(2.0f * x) + 1.0f);
@@ -58,7 +70,7 @@ __device__ __forceinline__ float logavg(float a, float b) {
 // ---------------------------------------------------------------- SP-WENO
 __device__ __forceinline__ float spw_fp(float t0, float t1) {
   const float eps = 1.0e-13f;
-  float psi = (fabsf(t0 - 1.0f) < eps) ? 0.0f : (1.0f - t1) * rcp(1.0f - t0);
+  float psi = (fabsf(t0 - 1.0f) < eps) ? 0.0f : quot(1.0f - t1, 1.0f - t0);
   return (fabsf(t0 - 1.0f) > eps && fabsf(psi + 1.0f) > eps) ? rcp(1.0f + psi) : 1.0f;
 }
 
@@ -69,7 +81,7 @@ __device__ __forceinline__ float spw_cp(float t0, float t1, float del1, float sc
   if (fabsf(t0 - 1.0f) < eps || fabsf(t1 - 1.0f) < eps)
     psi = 0.0f;
   else
-    psi = (1.0f - t1) * rcp(1.0f - t0);
+    psi = quot(1.0f - t1, 1.0f - t0);
   float C = 0.0f;
   corr = 0.0f;
   if (fabsf(psi + 1.0f) > eps && fabsf(t0 - 1.0f) > eps && psi < 0.0f) {
@@ -93,8 +105,7 @@ __device__ __forceinline__ float spw_cp(float t0, float t1, float del1, float sc
 __device__ __forceinline__ float spw_jump(float d0, float d1, float d2, float u1, float u2,
                                           int corr) {
   if (fabsf(d1) < 1.0e-13f) return d1;  // limiter.h:202-206: (u1, u2)
-  const float r = rcp(d1);
-  const float tp = d0 * r, tm = d2 * r;
+  const float tp = quot(d0, d1), tm = quot(d2, d1);
   const bool wc = corr == 1;
   const float scale = 0.5f * (fabsf(u1) + fabsf(u2));
   float c1c, c2c;
@@ -111,8 +122,8 @@ __device__ __forceinline__ float spw_jump(float d0, float d1, float d2, float u1
 // compute_vertices (neural_network.cc:135-260, vertex_mode 2, permutation 2)
 __device__ __forceinline__ void vertices(float tm, float tp, float maxd, float xv[5], float yv[5]) {
   const float thr = 1e-8f;
-  const float psi_p = (1.0f - tm) * rcp(1.0f - tp);
-  const float psi_m = rcp(psi_p);
+  const float psi_p = quot(1.0f - tm, 1.0f - tp);
+  const float psi_m = quot(1.0f, psi_p);
   const float g1 = fminf(maxd, 0.125f);
   const float g2 = -fminf(maxd, 0.375f);
   const float x1 = 0.125f * (1.0f + psi_p) - g1 * psi_p;
@@ -163,8 +174,7 @@ __device__ __forceinline__ void vertices(float tm, float tp, float maxd, float x
 __device__ __forceinline__ void features(float d0, float d1, float d2, float maxabs, float st[5],
                                          float xv[5], float yv[5]) {
   const float inv = rcp(fmaxf(1.0f, maxabs));
-  const float r = rcp(d1);
-  const float tm = d2 * r, tp = d0 * r;
+  const float tm = quot(d2, d1), tp = quot(d0, d1);
   const float a1 = fabsf(d0) * inv, a2 = fabsf(d1) * inv, a3 = fabsf(d2) * inv;
   st[0] = tanh(tm);
   st[1] = tanh(tp);
@@ -330,7 +340,7 @@ struct Euler2Mixed {
   }
 
   __device__ static void face(const Phys<float>& P, const void* ldsv, int NB, int base, int stride,
-                              int ax, double F[4]) {
+                              int ax, double F[4], unsigned& nfb) {
     const double* ld = (const double*)ldsv;
     const int rn = ax ? R_UY : R_UX, rt = ax ? R_UX : R_UY;
     double rho[4], un[4], ut[4], be[4], lr[4], lb[4];
@@ -398,24 +408,56 @@ struct Euler2Mixed {
     Z1[2] = (float)(Rt2 * Vt[1] + RE2 * Vl[1]);
     Z1[3] = (float)(S0 * Vf[1] + Rn4 * Vn[1] + Rt4 * Vt[1] + RE4 * Vl[1]);
     float jmp[4];
+#ifndef TECNO_HOST_BUILD
+    if constexpr (RV == RV_DSP && MM != MM_SIMT) {
+      // network on the matrix cores: every lane of the wave takes part in
+      // each of the 4 component evaluations
+      mfma::Net5<MM == MM_FP16> net;
+      net.load(P.mfma_w[MM == MM_FP16 ? 1 : 0], P.net.b, threadIdx.x & 63);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float d0 = dZ[i][0], d1 = dZ[i][1], d2 = dZ[i][2];
-      const float z1 = Z1[i], z2 = z1 + d1;
-      if (RV == RV_SPWENO || (RV == RV_GENERIC && P.recon != REC_DSP)) {
-        jmp[i] = fast::spw_jump(d0, d1, d2, z1, z2, P.corr);
-      } else {
-        const float z0 = z1 - d0, z3 = z2 + d2;
+      for (int i = 0; i < 4; ++i) {
+        const float d0 = dZ[i][0], d1 = dZ[i][1], d2 = dZ[i][2];
+        const float z1 = Z1[i], z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;
         const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
-        float st[5], xv[5], yv[5];
+        float st[5], xv[5], yv[5], lg[5];
         fast::features(d0, d1, d2, maxabs, st, xv, yv);
-        fast::body5(P.net, st);
-        float j = fast::head(st, xv, yv, d0, d1, d2);
-        if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
-        // sign property: fall back to SP-WENO where the network's jump has the
-        // wrong sign
-        if (P.sign_fallback && j * d1 < 0.0f) j = fast::spw_jump(d0, d1, d2, z1, z2, 0);
+        const bool skip = !(fabsf(d1) >= 1e-15f);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) st[k] = (skip || !(fabsf(st[k]) < INFINITY)) ? 0.0f : st[k];
+        net.eval(st, lg);
+        float j = fast::head(lg, xv, yv, d0, d1, d2);
+        if (skip) j = 0.0f;
+        if (P.sign_fallback && j * d1 < 0.0f) {
+          j = fast::spw_jump(d0, d1, d2, z1, z2, 0);
+          ++nfb;
+        }
         jmp[i] = j;
+      }
+    } else
+#endif
+    {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float d0 = dZ[i][0], d1 = dZ[i][1], d2 = dZ[i][2];
+        const float z1 = Z1[i], z2 = z1 + d1;
+        if (RV == RV_SPWENO || (RV == RV_GENERIC && P.recon != REC_DSP)) {
+          jmp[i] = fast::spw_jump(d0, d1, d2, z1, z2, P.corr);
+        } else {
+          const float z0 = z1 - d0, z3 = z2 + d2;
+          const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
+          float st[5], xv[5], yv[5];
+          fast::features(d0, d1, d2, maxabs, st, xv, yv);
+          fast::body5(P.net, st);
+          float j = fast::head(st, xv, yv, d0, d1, d2);
+          if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
+          // sign property: fall back to SP-WENO where the network's jump has
+          // the wrong sign
+          if (P.sign_fallback && j * d1 < 0.0f) {
+            j = fast::spw_jump(d0, d1, d2, z1, z2, 0);
+            ++nfb;
+          }
+          jmp[i] = j;
+        }
       }
     }
     // ---- dissipation R~ Lambda dZ (FP32) and the TeCNO flux (FP64)

@@ -22,7 +22,7 @@ struct ScalarPh {
   }
   // face flux: enflo_scalar num_flux.cc:12-57 and lin_/bur_ fluxes
   __device__ static void face(const Phys<T>& P, const void* ldsv, int NB, int base, int stride,
-                              int ax, T F[1]) {
+                              int ax, T F[1], unsigned& nfb) {
     const T* lds = (const T*)ldsv;
     T s[2 * G];
 #pragma unroll
@@ -192,7 +192,7 @@ struct Euler2Ph {
   // the dissipation is unchanged bit for bit.  Sums over the x and y
   // components keep the reference's x-before-y order via `ax`.
   __device__ static void face(const Phys<T>& P, const void* ldsv, int NB, int base, int stride,
-                              int ax, T F[4]) {
+                              int ax, T F[4], unsigned& nfb) {
     const T* lds = (const T*)ldsv;
     constexpr int S = 2 * G;
     const int rn = ax ? E_UY : E_UX, rt = ax ? E_UX : E_UY;

@@ -11,8 +11,8 @@ namespace tecno {
 
 // Neural_Network::forward on n raw stencils with the 5-5-5-5-5 network on
 // matrix cores (bf16 or fp16 operands, fp32 accumulation).
-hipError_t mlp_mfma_batch(hipStream_t s, const float* W, const float* b, int nl, bool fp16,
-                          const double* u4, int64_t n, double* l, double* r);
+hipError_t mlp_mfma_batch(hipStream_t s, const unsigned short* packedA, const float* bias,
+                          bool fp16, const double* u4, int64_t n, double* l, double* r);
 
 // Training state on the device: fp32 master weights, Adam moments.
 class MlpTrainer {

@@ -1,0 +1,75 @@
+// mlp_mfma.hpp — the DSP-WENO 5-5-5-5-5 network on CDNA4 matrix cores.
+//
+// One v_mfma_f32_32x32x16_{bf16,f16} per layer evaluates the network for the
+// 64 lanes of a wave at once.  Two stencils share each column of the 32x32
+// product, placed block-diagonally so that no lane exchanges data:
+//   lane n      (n < 32): inputs at k = 0..4,   outputs at rows 0,1,2,3,8
+//   lane n + 32          : inputs at k = 8..12,  outputs at rows 4,5,6,7,12
+// With the CDNA 32x32x16 operand layout (lane l holds B[8*(l/32)+i][l%32] and
+// D rows 8*(s/4) + 4*(l/32) + s%4 at accumulator slot s), every lane finds its
+// own stencil's outputs in accumulator slots 0..4, which are directly the
+// next layer's inputs.  The bias enters as the C operand (b0..b4 in slots
+// 0..4 of every lane).  A (the weights) is packed on the host per lane.
+#pragma once
+#ifndef TECNO_HOST_BUILD
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+namespace tecno {
+namespace mfma {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <bool H>
+struct Net5 {
+  using E = std::conditional_t<H, _Float16, __bf16>;
+  using V = std::conditional_t<H, f16x8, bf16x8>;
+  V A[4];
+  float b[4][5];
+
+  // packed: [layer][lane 0..63][8] weights of this lane's row block
+  __device__ void load(const unsigned short* packed, const float* bias, int lane) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const u16x8 w = *(const u16x8*)(packed + ((size_t)l * 64 + lane) * 8);
+      A[l] = __builtin_bit_cast(V, w);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) b[l][j] = bias[l * 16 + j];
+    }
+  }
+  __device__ static V pack(float x0, float x1, float x2, float x3, float x4) {
+    V v = {(E)x0, (E)x1, (E)x2, (E)x3, (E)x4, (E)0.0f, (E)0.0f, (E)0.0f};
+    return v;
+  }
+  __device__ static f32x16 mma(V a, V x, f32x16 c) {
+    if constexpr (H)
+      return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, x, c, 0, 0, 0);
+    else
+      return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, x, c, 0, 0, 0);
+  }
+  // all 64 lanes of the wave must call this together
+  __device__ void eval(const float in[5], float out[5]) const {
+    V x = pack(in[0], in[1], in[2], in[3], in[4]);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      f32x16 c = {b[l][0], b[l][1], b[l][2], b[l][3], b[l][4], 0.f, 0.f, 0.f,
+                  0.f,     0.f,     0.f,     0.f,     0.f,     0.f, 0.f, 0.f};
+      const f32x16 d = mma(A[l], x, c);
+      if (l < 3) {
+        x = pack(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f), fmaxf(d[2], 0.f), fmaxf(d[3], 0.f),
+                 fmaxf(d[4], 0.f));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 5; ++j) out[j] = d[j];
+      }
+    }
+  }
+};
+
+}  // namespace mfma
+}  // namespace tecno
+#endif

@@ -15,7 +15,7 @@ import pytest
 
 import oracle_ctypes as orc
 from cases import case, collapse, euler_cfg, euler_prim_for, fixtures, scalar_cfg
-from tecno import FP32, FP64, Solver, make_cfg, MLP_FP32, MLP_FP64
+from tecno import FP32, FP64, Solver, make_cfg, MLP_BF16, MLP_FP16, MLP_FP32, MLP_FP64
 
 pytestmark = pytest.mark.gpu
 
@@ -248,6 +248,55 @@ def test_mlp_forward_seam_vs_oracle():
     jump = st[:, 2] - st[:, 1]
     ok = (jump * (r - l) >= 0) | (np.abs(r - l) <= 1e-6 * scale)
     assert ok.all()
+
+
+@pytest.mark.parametrize("mode,tol", [(MLP_BF16, 0.008), (MLP_FP16, 0.001)])
+def test_mlp_mfma_seam_vs_oracle(mode, tol):
+    """Matrix-core network (bf16 / fp16 operands, fp32 accumulation): the
+    reconstructed values differ from the FP64 network by at most tol times the
+    stencil's total variation for 99% of stencils and 3 tol for all of them
+    (8 and 11 significant bits in the operands), and every result keeps the
+    sign property (convex vertex weights are feasible whatever the network
+    outputs)."""
+    st = case("point", "recon_stencils")["st"]
+    st = st[np.abs(st[:, 2] - st[:, 1]) > 1e-12]
+    lo, ro = orc.nn_forward(orc.load_net(), st)
+    s = Solver(make_cfg(system="scalar", flux="lin_tecno4", recon="sp_weno_dl", n=(8, 2, 2), dims=1,
+                        mlp_dtype=mode, precision=FP32))
+    s.load_net(NET_FILE)
+    l, r = s.mlp_forward_batch(st)
+    tv = np.abs(np.diff(st, axis=1)).sum(axis=1) + 1e-300
+    err = np.maximum(np.abs(l - lo), np.abs(r - ro)) / tv
+    assert np.isfinite(l).all() and np.isfinite(r).all()
+    assert np.mean(err <= tol) >= 0.99, np.quantile(err, [0.5, 0.9, 0.99])
+    assert err.max() <= 3 * tol, err.max()
+    jump = st[:, 2] - st[:, 1]
+    scale = 1.0 + np.abs(st).max(axis=1)
+    assert ((jump * (r - l) >= 0) | (np.abs(r - l) <= 1e-6 * scale)).all()
+
+
+@pytest.mark.parametrize("mode,tol99", [(MLP_BF16, 1e-5), (MLP_FP16, 5e-7)])
+def test_mfma_stage_close_to_simt(mode, tol99):
+    """The stage kernel with the network on matrix cores tracks the FP32 SIMT
+    network over 10 steps of a smooth flow: per cell, the largest relative
+    difference over the variables is below tol99 for 99% of the cells and
+    below 1e-4 everywhere (bf16 / fp16 operands carry 8 / 11 significant
+    bits; the few cells above tol99 sit where the network output is steep)."""
+    rng = np.random.default_rng(12)
+    nx, ny = 64, 48
+    prim = _random_euler_state(rng, nx, ny, True)
+    out = {}
+    for m in (MLP_FP32, mode):
+        s = solver_for(make_cfg(n=(nx, ny, 2), dims=2, recon="sp_weno_dl", bc="periodic", precision=FP32,
+                                mlp_dtype=m, sign_fallback=1), True)
+        s.set_state(prim)
+        s.run_steps(10)
+        out[m] = s.get_state()
+    a, b = out[mode], out[MLP_FP32]
+    d = (np.abs(a - b) / np.abs(b).max(axis=-1, keepdims=True)).max(axis=0)
+    assert np.isfinite(a).all()
+    assert np.quantile(d, 0.99) <= tol99, np.quantile(d, [0.5, 0.99, 1.0])
+    assert d.max() <= 1e-4, d.max()
 
 
 # ---------------------------------------------------------------- invariants

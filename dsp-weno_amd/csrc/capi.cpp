@@ -8,6 +8,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <random>
 #include <vector>
 
 #include "../../include/tecno.h"
@@ -200,10 +201,11 @@ static int upload_net(tecno_ctx* c) {
           int j = -1;  // output neuron of row m for this lane's half
           if (half == 0 && (m < 4 || m == 8)) j = m < 4 ? m : 4;
           if (half == 1 && ((m >= 4 && m < 8) || m == 12)) j = m < 8 ? m - 4 : 4;
-          for (int e = 0; e < 5; ++e) {
-            const float w = j < 0 ? 0.0f
-                                  : (float)c->net.W[(size_t)l * TECNO_NET_MAXW * TECNO_NET_MAXW +
-                                                    j * TECNO_NET_MAXW + e];
+          for (int e = 0; e < 6; ++e) {  // 5 weights, then the bias (input 1 at slot 5)
+            const float w = j < 0   ? 0.0f
+                            : e < 5 ? (float)c->net.W[(size_t)l * TECNO_NET_MAXW * TECNO_NET_MAXW +
+                                                      j * TECNO_NET_MAXW + e]
+                                    : (float)c->net.b[(size_t)l * TECNO_NET_MAXW + j];
             uint32_t u;
             std::memcpy(&u, &w, 4);
             u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
@@ -778,8 +780,8 @@ int tecno_mlp_forward_batch(tecno_ctx* c, const double* u4, int64_t n, double* l
   if (c->cfg.mlp_dtype == TECNO_MLP_BF16 || c->cfg.mlp_dtype == TECNO_MLP_FP16)
     return with_device_io(c, {{u4, (size_t)n * 4}}, {{l, (size_t)n}, {r, (size_t)n}},
                           [&](std::vector<double*>& i, std::vector<double*>& o) {
-                            return mlp_mfma_batch(c->stream, c->P32.mfma_w[c->cfg.mlp_dtype == TECNO_MLP_FP16 ? 1 : 0],
-                                                  c->db32, c->cfg.mlp_dtype == TECNO_MLP_FP16, i[0], n,
+                            const bool h = c->cfg.mlp_dtype == TECNO_MLP_FP16;
+                            return mlp_mfma_batch(c->stream, c->P32.mfma_w[h ? 1 : 0], h, i[0], n,
                                                   o[0], o[1]);
                           });
   return with_device_io(c, {{u4, (size_t)n * 4}}, {{l, (size_t)n}, {r, (size_t)n}},
@@ -905,9 +907,76 @@ int tecno_train_grad(tecno_ctx* c, const double* rows, int64_t n, int use_bf16, 
   return 0;
 }
 
+int tecno_train_set_data(tecno_ctx* c, const double* rows, int64_t n) {
+  if (!c || !rows) return fail(TECNO_EINVAL, "null argument");
+  if (!c->trainer) return fail(TECNO_ESTATE, "tecno_train_init first");
+  std::string err;
+  if (c->trainer->set_data(rows, n, err)) return fail(TECNO_EINVAL, err);
+  return 0;
+}
+
+int tecno_train_epoch(tecno_ctx* c, int64_t batch, uint64_t seed, int use_bf16, double* loss) {
+  if (!c) return fail(TECNO_EINVAL, "null argument");
+  if (!c->trainer) return fail(TECNO_ESTATE, "tecno_train_init first");
+  std::string err;
+  double l = 0;
+  if (c->trainer->epoch(batch, seed, use_bf16 != 0, &l, err)) return fail(TECNO_EHIP, err);
+  if (loss) *loss = l;
+  return sync_net_from_trainer(c);
+}
+
 int tecno_make_train_rows(const double* u4, const double* target, int64_t n, double* rows) {
   if (!u4 || !target || !rows) return fail(TECNO_EINVAL, "null argument");
-  return make_train_rows_host(u4, target, n, rows);
+  if (n <= 0) return 0;
+  std::string err;
+  if (make_train_rows(nullptr, u4, target, n, rows, err)) return fail(TECNO_EHIP, err);
+  return 0;
+}
+
+int tecno_save_net(const tecno_ctx* c, const char* path) {
+  if (!c || !path) return fail(TECNO_EINVAL, "null argument");
+  if (!c->has_net) return fail(TECNO_ESTATE, "no network loaded");
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(TECNO_EIO, std::string("cannot write ") + path);
+  std::fprintf(f, "Weights\n");
+  for (int l = 0; l < c->net.nl; ++l) {
+    std::fprintf(f, "Layer %d\nBEGIN\n", l + 1);
+    for (int o = 0; o < c->net.w[l + 1]; ++o) {
+      for (int i = 0; i < c->net.w[l]; ++i)
+        std::fprintf(f, i ? " %.18e" : "%.18e",
+                     c->net.W[(size_t)l * TECNO_NET_MAXW * TECNO_NET_MAXW + o * TECNO_NET_MAXW + i]);
+      std::fprintf(f, "\n");
+    }
+    std::fprintf(f, "END\n");
+  }
+  std::fprintf(f, "Biases\n");
+  for (int l = 0; l < c->net.nl; ++l) {
+    std::fprintf(f, "Layer %d\nBEGIN\n", l + 1);
+    for (int o = 0; o < c->net.w[l + 1]; ++o)
+      std::fprintf(f, "%.18e\n", c->net.b[(size_t)l * TECNO_NET_MAXW + o]);
+    std::fprintf(f, "END\n");
+  }
+  const bool ok = std::fclose(f) == 0;
+  return ok ? 0 : fail(TECNO_EIO, std::string("write failed: ") + path);
+}
+
+int tecno_init_net(tecno_ctx* c, int nl, const int32_t* widths, uint64_t seed) {
+  if (!c || !widths) return fail(TECNO_EINVAL, "null argument");
+  if (nl < 1 || nl > TECNO_NET_MAXL) return fail(TECNO_EINVAL, "bad layer count");
+  if (widths[0] != 5 || widths[nl] != 5)
+    return fail(TECNO_EINVAL, "the network maps 5 inputs to 5 vertex weights");
+  std::mt19937_64 rng(seed);
+  std::vector<double> W, b;
+  for (int l = 0; l < nl; ++l) {
+    const int ni = widths[l], no = widths[l + 1];
+    if (ni < 1 || no < 1 || ni > TECNO_NET_MAXW || no > TECNO_NET_MAXW)
+      return fail(TECNO_EINVAL, "layer widths must be 1..16");
+    const double aw = std::sqrt(6.0 / (double)(ni + no)), ab = 1.0 / std::sqrt((double)ni);
+    std::uniform_real_distribution<double> dw(-aw, aw), db(-ab, ab);
+    for (int q = 0; q < ni * no; ++q) W.push_back(dw(rng));
+    for (int q = 0; q < no; ++q) b.push_back(db(rng));
+  }
+  return tecno_set_net(c, nl, widths, W.data(), b.data());
 }
 
 }  // extern "C"

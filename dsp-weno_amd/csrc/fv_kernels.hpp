@@ -197,8 +197,8 @@ struct Tile {
 
 #ifndef TECNO_HOST_BUILD
 template <class PH, int DIMS, int TX, int TY, int G>
-__global__ __launch_bounds__(256) void stage_kernel(StageArgs<typename PH::TS> A,
-                                                    Phys<typename PH::TP> P, GridDev g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PH::MIN_WAVES)))
+void stage_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDev g) {
   using TL = Tile<PH, DIMS, TX, TY, G>;
   using TS = typename PH::TS;
   __shared__ __attribute__((aligned(16))) unsigned char lds[TL::LDS_BYTES];

@@ -253,6 +253,18 @@ __device__ __forceinline__ void body5(const NetView<float>& net, float st[5]) {
 
 }  // namespace fast
 
+// no instruction is moved across this point by the machine scheduler
+__device__ __forceinline__ void sched_fence() {
+#ifndef TECNO_HOST_BUILD
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
+// x[i] for a lane-varying i without indexing a register array
+__device__ __forceinline__ float sel4(int i, float a, float b, float c, float d) {
+  return i == 0 ? a : i == 1 ? b : i == 2 ? c : d;
+}
+
 // ---------------------------------------------------------------- FP64 helpers
 #ifdef TECNO_HOST_BUILD
 __device__ __forceinline__ double rcp64(double x) { return 1.0 / x; }
@@ -289,6 +301,10 @@ enum { R_RHO = 0, R_UX, R_UY, R_BETA, R_LRHO, R_LBETA, R_VF, R_N };
 template <int RV, int MM>
 struct Euler2Mixed {
   using TS = double;
+  // waves per SIMD the register budget is cut to: two for the network paths
+  // (latency hiding of the matrix-core and transcendental chains without
+  // spilling), the compiler's choice otherwise
+  static constexpr int MIN_WAVES = RV == RV_DSP ? 2 : 1;
   using TF = double;
   using TP = float;
   static constexpr int G = 2;
@@ -343,81 +359,95 @@ struct Euler2Mixed {
                               int ax, double F[4], unsigned& nfb) {
     const double* ld = (const double*)ldsv;
     const int rn = ax ? R_UY : R_UX, rt = ax ? R_UX : R_UY;
-    double rho[4], un[4], ut[4], be[4], lr[4], lb[4];
-#pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int c = base + l * stride;
-      rho[l] = ld[R_RHO * NB + c];
-      un[l] = ld[rn * NB + c];
-      ut[l] = ld[rt * NB + c];
-      be[l] = ld[R_BETA * NB + c];
-      lr[l] = ld[R_LRHO * NB + c];
-      lb[l] = ld[R_LBETA * NB + c];
+    // Order of work keeps the register footprint small: the eigensystem and
+    // the projected jumps first, then the FP32 reconstruction with only float
+    // copies of the eigenvectors live, then the FP64 EC4 flux from the
+    // records re-read out of LDS.
+    // ---- central two-point flux F(1,2) and its means (kepec.cc:10-48)
+    double Fc[4], aux[5];
+    {
+      const int c1 = base + stride, c2 = base + 2 * stride;
+      kepec(P, ld[R_RHO * NB + c1], ld[rn * NB + c1], ld[rt * NB + c1], ld[R_BETA * NB + c1],
+            ld[R_LRHO * NB + c1], ld[R_LBETA * NB + c1], ld[R_RHO * NB + c2], ld[rn * NB + c2],
+            ld[rt * NB + c2], ld[R_BETA * NB + c2], ld[R_LRHO * NB + c2], ld[R_LBETA * NB + c2], Fc,
+            aux);
     }
-    // ---- entropy-conservative flux, 4th order (kepec4.cc:10-38), FP64
-    double Fe[4], f0[4], aux[5];
-    kepec(P, rho[1], un[1], ut[1], be[1], lr[1], lb[1], rho[2], un[2], ut[2], be[2], lr[2], lb[2],
-          Fe, aux);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Fe[v] *= (4.0 / 3.0);
-    kepec(P, rho[0], un[0], ut[0], be[0], lr[0], lb[0], rho[2], un[2], ut[2], be[2], lr[2], lb[2],
-          f0);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Fe[v] -= (1.0 / 6.0) * f0[v];
-    kepec(P, rho[1], un[1], ut[1], be[1], lr[1], lb[1], rho[3], un[3], ut[3], be[3], lr[3], lb[3],
-          f0);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Fe[v] -= (1.0 / 6.0) * f0[v];
-
     // ---- eigensystem at the face (kepes_tecno4_roe.cc:39-113) and the
     // projection of the entropy-variable jumps, FP64: a jump of a scaled
     // variable can be a near-total cancellation of its terms (e.g. the shear
     // wave at a symmetric extremum), and the zero-jump tests (1e-13 / 1e-15)
     // are absolute
-    const double rhoh = aux[0], betah = aux[1], ph = aux[2], vn = aux[3], vt = aux[4];
-    const double a = sqrt(0.5 * P.gamma_d * rcp64(betah));
-    const double vsq = vn * vn + vt * vt;
-    const double H = a * a * P.inv_gm1_d + 0.5 * vsq;
-    const double c1 = sqrt(rhoh * P.inv_gamma_d);
-    const double S0 = P.s05_d * c1, S1 = P.sgm1_d * c1, S2 = sqrt(ph);
-    const double Rn0 = (vn - a) * S0, Rt0 = vt * S0, RE0 = (H - a * vn) * S0;
-    const double Rn1 = vn * S1, Rt1 = vt * S1, RE1 = (0.5 * vsq) * S1;
-    const double Rt2 = -S2, RE2 = -vt * S2;
-    const double Rn4 = (vn + a) * S0, Rt4 = Rt0, RE4 = (H + a * vn) * S0;
-    double Vf[4], Vn[4], Vt[4], Vl[4];
+    float dZ[4][3], Z1[4];
+    float vnf, af, s0, s1, Rn0f, Rt0f, RE0f, Rn1f, Rt1f, RE1f, Rt2f, RE2f, Rn4f, RE4f;
+    {
+      const double rhoh = aux[0], betah = aux[1], ph = aux[2], vn = aux[3], vt = aux[4];
+      const double a = sqrt(0.5 * P.gamma_d * rcp64(betah));
+      const double vsq = vn * vn + vt * vt;
+      const double H = a * a * P.inv_gm1_d + 0.5 * vsq;
+      const double c1 = sqrt(rhoh * P.inv_gamma_d);
+      const double S0 = P.s05_d * c1, S1 = P.sgm1_d * c1, S2 = sqrt(ph);
+      const double Rn0 = (vn - a) * S0, Rt0 = vt * S0, RE0 = (H - a * vn) * S0;
+      const double Rn1 = vn * S1, Rt1 = vt * S1, RE1 = (0.5 * vsq) * S1;
+      const double Rt2 = -S2, RE2 = -vt * S2;
+      const double Rn4 = (vn + a) * S0, Rt4 = Rt0, RE4 = (H + a * vn) * S0;
+      double Vf[4], Vn[4], Vt[4], Vl[4];
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      Vf[l] = ld[R_VF * NB + base + l * stride];
-      Vn[l] = 2.0 * be[l] * un[l];
-      Vt[l] = 2.0 * be[l] * ut[l];
-      Vl[l] = -2.0 * be[l];
-    }
-    float dZ[4][3];
+      for (int l = 0; l < 4; ++l) {
+        const int cc = base + l * stride;
+        const double be = ld[R_BETA * NB + cc];
+        Vf[l] = ld[R_VF * NB + cc];
+        Vn[l] = 2.0 * be * ld[rn * NB + cc];
+        Vt[l] = 2.0 * be * ld[rt * NB + cc];
+        Vl[l] = -2.0 * be;
+      }
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const double df = Vf[j + 1] - Vf[j], dn = Vn[j + 1] - Vn[j];
-      const double dt = Vt[j + 1] - Vt[j], dl = Vl[j + 1] - Vl[j];
-      dZ[0][j] = (float)(S0 * df + Rn0 * dn + Rt0 * dt + RE0 * dl);
-      dZ[1][j] = (float)(S1 * df + Rn1 * dn + Rt1 * dt + RE1 * dl);
-      dZ[2][j] = (float)(Rt2 * dt + RE2 * dl);
-      dZ[3][j] = (float)(S0 * df + Rn4 * dn + Rt4 * dt + RE4 * dl);
+      for (int j = 0; j < 3; ++j) {
+        const double df = Vf[j + 1] - Vf[j], dn = Vn[j + 1] - Vn[j];
+        const double dt = Vt[j + 1] - Vt[j], dl = Vl[j + 1] - Vl[j];
+        dZ[0][j] = (float)(S0 * df + Rn0 * dn + Rt0 * dt + RE0 * dl);
+        dZ[1][j] = (float)(S1 * df + Rn1 * dn + Rt1 * dt + RE1 * dl);
+        dZ[2][j] = (float)(Rt2 * dt + RE2 * dl);
+        dZ[3][j] = (float)(S0 * df + Rn4 * dn + Rt4 * dt + RE4 * dl);
+      }
+      Z1[0] = (float)(S0 * Vf[1] + Rn0 * Vn[1] + Rt0 * Vt[1] + RE0 * Vl[1]);
+      Z1[1] = (float)(S1 * Vf[1] + Rn1 * Vn[1] + Rt1 * Vt[1] + RE1 * Vl[1]);
+      Z1[2] = (float)(Rt2 * Vt[1] + RE2 * Vl[1]);
+      Z1[3] = (float)(S0 * Vf[1] + Rn4 * Vn[1] + Rt4 * Vt[1] + RE4 * Vl[1]);
+      vnf = (float)vn;
+      af = (float)a;
+      s0 = (float)S0;
+      s1 = (float)S1;
+      Rn0f = (float)Rn0;
+      Rt0f = (float)Rt0;
+      RE0f = (float)RE0;
+      Rn1f = (float)Rn1;
+      Rt1f = (float)Rt1;
+      RE1f = (float)RE1;
+      Rt2f = (float)Rt2;
+      RE2f = (float)RE2;
+      Rn4f = (float)Rn4;
+      RE4f = (float)RE4;
     }
-    float Z1[4];
-    Z1[0] = (float)(S0 * Vf[1] + Rn0 * Vn[1] + Rt0 * Vt[1] + RE0 * Vl[1]);
-    Z1[1] = (float)(S1 * Vf[1] + Rn1 * Vn[1] + Rt1 * Vt[1] + RE1 * Vl[1]);
-    Z1[2] = (float)(Rt2 * Vt[1] + RE2 * Vl[1]);
-    Z1[3] = (float)(S0 * Vf[1] + Rn4 * Vn[1] + Rt4 * Vt[1] + RE4 * Vl[1]);
-    float jmp[4];
+    // DSP: keep the scheduler from hoisting the later LDS re-reads (and their
+    // registers) across the network evaluation
+    if constexpr (RV == RV_DSP) sched_fence();
+    float jmp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    unsigned bad = 0;  // components whose network jump fails the sign property
 #ifndef TECNO_HOST_BUILD
     if constexpr (RV == RV_DSP && MM != MM_SIMT) {
       // network on the matrix cores: every lane of the wave takes part in
       // each of the 4 component evaluations
       mfma::Net5<MM == MM_FP16> net;
-      net.load(P.mfma_w[MM == MM_FP16 ? 1 : 0], P.net.b, threadIdx.x & 63);
-#pragma unroll
+      net.load(P.mfma_w[MM == MM_FP16 ? 1 : 0], threadIdx.x & 63);
+      // components one after the other (not unrolled): only one component's
+      // features, vertices and activations are live at a time
+#pragma unroll 1
       for (int i = 0; i < 4; ++i) {
-        const float d0 = dZ[i][0], d1 = dZ[i][1], d2 = dZ[i][2];
-        const float z1 = Z1[i], z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;
+        const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
+        const float d1 = sel4(i, dZ[0][1], dZ[1][1], dZ[2][1], dZ[3][1]);
+        const float d2 = sel4(i, dZ[0][2], dZ[1][2], dZ[2][2], dZ[3][2]);
+        const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
+        const float z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;
         const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
         float st[5], xv[5], yv[5], lg[5];
         fast::features(d0, d1, d2, maxabs, st, xv, yv);
@@ -427,15 +457,32 @@ struct Euler2Mixed {
         net.eval(st, lg);
         float j = fast::head(lg, xv, yv, d0, d1, d2);
         if (skip) j = 0.0f;
-        if (P.sign_fallback && j * d1 < 0.0f) {
-          j = fast::spw_jump(d0, d1, d2, z1, z2, 0);
-          ++nfb;
-        }
-        jmp[i] = j;
+        bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) jmp[k] = (k == i) ? j : jmp[k];
       }
     } else
 #endif
-    {
+    if constexpr (RV == RV_DSP) {
+      // FP32 network on the vector units, components one after the other
+#pragma unroll 1
+      for (int i = 0; i < 4; ++i) {
+        const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
+        const float d1 = sel4(i, dZ[0][1], dZ[1][1], dZ[2][1], dZ[3][1]);
+        const float d2 = sel4(i, dZ[0][2], dZ[1][2], dZ[2][2], dZ[3][2]);
+        const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
+        const float z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;
+        const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
+        float st[5], xv[5], yv[5];
+        fast::features(d0, d1, d2, maxabs, st, xv, yv);
+        fast::body5(P.net, st);
+        float j = fast::head(st, xv, yv, d0, d1, d2);
+        if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
+        bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) jmp[k] = (k == i) ? j : jmp[k];
+      }
+    } else {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float d0 = dZ[i][0], d1 = dZ[i][1], d2 = dZ[i][2];
@@ -450,26 +497,51 @@ struct Euler2Mixed {
           fast::body5(P.net, st);
           float j = fast::head(st, xv, yv, d0, d1, d2);
           if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
-          // sign property: fall back to SP-WENO where the network's jump has
-          // the wrong sign
-          if (P.sign_fallback && j * d1 < 0.0f) {
-            j = fast::spw_jump(d0, d1, d2, z1, z2, 0);
-            ++nfb;
-          }
+          bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
           jmp[i] = j;
         }
       }
     }
-    // ---- dissipation R~ Lambda dZ (FP32) and the TeCNO flux (FP64)
-    const float vnf = (float)vn, af = (float)a;
+    // sign property: SP-WENO replaces the network's jump where its sign
+    // disagrees with the cell jump.  One component per pass, so a wave pays
+    // for the largest count of one lane rather than for every component that
+    // any of its lanes needs.
+    if (!P.sign_fallback) bad = 0;
+    while (bad) {
+      const int i = __builtin_ctz(bad);
+      bad &= bad - 1;
+      const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
+      const float d1 = sel4(i, dZ[0][1], dZ[1][1], dZ[2][1], dZ[3][1]);
+      const float d2 = sel4(i, dZ[0][2], dZ[1][2], dZ[2][2], dZ[3][2]);
+      const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
+      const float j = fast::spw_jump(d0, d1, d2, z1, z1 + d1, 0);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) jmp[k] = (k == i) ? j : jmp[k];
+      ++nfb;
+    }
+    // ---- dissipation R~ Lambda dZ (FP32)
     const float L1 = fabsf(vnf), L0 = fabsf(vnf - af), L4 = fabsf(vnf + af);
-    const float s0 = (float)S0, s1 = (float)S1;
     const float Dr = s0 * L0 * jmp[0] + s1 * L1 * jmp[1] + s0 * L4 * jmp[3];
-    const float Dn = (float)Rn0 * L0 * jmp[0] + (float)Rn1 * L1 * jmp[1] + (float)Rn4 * L4 * jmp[3];
-    const float Dt = (float)Rt0 * L0 * jmp[0] + (float)Rt1 * L1 * jmp[1] + (float)Rt2 * L1 * jmp[2] +
-                     (float)Rt4 * L4 * jmp[3];
-    const float DE = (float)RE0 * L0 * jmp[0] + (float)RE1 * L1 * jmp[1] + (float)RE2 * L1 * jmp[2] +
-                     (float)RE4 * L4 * jmp[3];
+    const float Dn = Rn0f * L0 * jmp[0] + Rn1f * L1 * jmp[1] + Rn4f * L4 * jmp[3];
+    const float Dt = Rt0f * L0 * jmp[0] + Rt1f * L1 * jmp[1] + Rt2f * L1 * jmp[2] +
+                     Rt0f * L4 * jmp[3];
+    const float DE = RE0f * L0 * jmp[0] + RE1f * L1 * jmp[1] + RE2f * L1 * jmp[2] +
+                     RE4f * L4 * jmp[3];
+    if constexpr (RV == RV_DSP) sched_fence();
+    // ---- entropy-conservative flux, 4th order (kepec4.cc:10-38), FP64
+    double Fe[4], f0[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fe[v] = (4.0 / 3.0) * Fc[v];
+#pragma unroll
+    for (int pr = 0; pr < 2; ++pr) {
+      const int ca = base + pr * stride, cb = base + (pr + 2) * stride;  // (0,2), (1,3)
+      kepec(P, ld[R_RHO * NB + ca], ld[rn * NB + ca], ld[rt * NB + ca], ld[R_BETA * NB + ca],
+            ld[R_LRHO * NB + ca], ld[R_LBETA * NB + ca], ld[R_RHO * NB + cb], ld[rn * NB + cb],
+            ld[rt * NB + cb], ld[R_BETA * NB + cb], ld[R_LRHO * NB + cb], ld[R_LBETA * NB + cb], f0);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Fe[v] -= (1.0 / 6.0) * f0[v];
+    }
+    // ---- TeCNO flux
     const double area = P.area_d[ax];
     const double Fn = (Fe[1] - 0.5 * (double)Dn) * area;
     const double Ft = (Fe[2] - 0.5 * (double)Dt) * area;

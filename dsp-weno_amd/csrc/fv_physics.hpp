@@ -11,6 +11,7 @@ namespace tecno {
 template <typename T, int G>
 struct ScalarPh {
   using TS = T;  // state in HBM
+  static constexpr int MIN_WAVES = 1;
   using TF = T;  // face flux
   using TP = T;  // parameters
   static constexpr int NV = 1;
@@ -174,6 +175,7 @@ enum { RV_SPWENO = 0, RV_DSP = 1, RV_GENERIC = 2 };
 template <typename T, int G, int FV, int RV>
 struct Euler2Ph {
   using TS = T;
+  static constexpr int MIN_WAVES = 1;
   using TF = T;
   using TP = T;
   static constexpr int NV = 4, NREC = E_NREC;

@@ -8,8 +8,11 @@
 // With the CDNA 32x32x16 operand layout (lane l holds B[8*(l/32)+i][l%32] and
 // D rows 8*(s/4) + 4*(l/32) + s%4 at accumulator slot s), every lane finds its
 // own stencil's outputs in accumulator slots 0..4, which are directly the
-// next layer's inputs.  The bias enters as the C operand (b0..b4 in slots
-// 0..4 of every lane).  A (the weights) is packed on the host per lane.
+// next layer's inputs.  The bias rides in the weights: each lane's input
+// block carries a constant 1 at k = 5 (k = 13) and the matching weight is the
+// bias, so the accumulator input C is the inline constant 0 and no register
+// block has to be initialised per product.  A (the weights) is packed on the
+// host per lane.
 #pragma once
 #ifndef TECNO_HOST_BUILD
 #include <hip/hip_runtime.h>
@@ -29,20 +32,17 @@ struct Net5 {
   using E = std::conditional_t<H, _Float16, __bf16>;
   using V = std::conditional_t<H, f16x8, bf16x8>;
   V A[4];
-  float b[4][5];
 
-  // packed: [layer][lane 0..63][8] weights of this lane's row block
-  __device__ void load(const unsigned short* packed, const float* bias, int lane) {
+  // packed: [layer][lane 0..63][8] weights of this lane's row block, bias at 5
+  __device__ void load(const unsigned short* packed, int lane) {
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
       const u16x8 w = *(const u16x8*)(packed + ((size_t)l * 64 + lane) * 8);
       A[l] = __builtin_bit_cast(V, w);
-#pragma unroll
-      for (int j = 0; j < 5; ++j) b[l][j] = bias[l * 16 + j];
     }
   }
   __device__ static V pack(float x0, float x1, float x2, float x3, float x4) {
-    V v = {(E)x0, (E)x1, (E)x2, (E)x3, (E)x4, (E)0.0f, (E)0.0f, (E)0.0f};
+    V v = {(E)x0, (E)x1, (E)x2, (E)x3, (E)x4, (E)1.0f, (E)0.0f, (E)0.0f};
     return v;
   }
   __device__ static f32x16 mma(V a, V x, f32x16 c) {
@@ -54,11 +54,10 @@ struct Net5 {
   // all 64 lanes of the wave must call this together
   __device__ void eval(const float in[5], float out[5]) const {
     V x = pack(in[0], in[1], in[2], in[3], in[4]);
+    const f32x16 zero = {};
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
-      f32x16 c = {b[l][0], b[l][1], b[l][2], b[l][3], b[l][4], 0.f, 0.f, 0.f,
-                  0.f,     0.f,     0.f,     0.f,     0.f,     0.f, 0.f, 0.f};
-      const f32x16 d = mma(A[l], x, c);
+      const f32x16 d = mma(A[l], x, zero);
       if (l < 3) {
         x = pack(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f), fmaxf(d[2], 0.f), fmaxf(d[3], 0.f),
                  fmaxf(d[4], 0.f));

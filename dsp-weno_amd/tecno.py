@@ -150,6 +150,10 @@ def lib():
         L.tecno_train_step.argtypes = [P, dp, I64, I32, dp]
         L.tecno_train_grad.argtypes = [P, dp, I64, I32, dp, dp]
         L.tecno_make_train_rows.argtypes = [dp, dp, I64, dp]
+        L.tecno_train_set_data.argtypes = [P, dp, I64]
+        L.tecno_train_epoch.argtypes = [P, I64, C.c_uint64, I32, dp]
+        L.tecno_save_net.argtypes = [P, C.c_char_p]
+        L.tecno_init_net.argtypes = [P, I32, C.POINTER(C.c_int32), C.c_uint64]
         L.tecno_timing_enable.argtypes = [P, I32]
         L.tecno_timing_get.argtypes = [P, dp, C.POINTER(C.c_int64)]
         L.tecno_last_error.restype = C.c_char_p
@@ -306,6 +310,22 @@ class Solver:
         _check(lib().tecno_train_grad(self.h, _dp(rows), rows.shape[0], int(use_bf16), C.byref(loss), _dp(g)))
         return loss.value, g
 
+    def train_set_data(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        _check(lib().tecno_train_set_data(self.h, _dp(rows), rows.shape[0]))
+
+    def train_epoch(self, batch, seed, use_bf16=True):
+        loss = C.c_double()
+        _check(lib().tecno_train_epoch(self.h, int(batch), int(seed), int(use_bf16), C.byref(loss)))
+        return loss.value
+
+    def save_net(self, path):
+        _check(lib().tecno_save_net(self.h, str(path).encode()))
+
+    def init_net(self, widths, seed):
+        w = (C.c_int32 * len(widths))(*[int(x) for x in widths])
+        _check(lib().tecno_init_net(self.h, len(widths) - 1, w, int(seed)))
+
     def timing(self, on=True):
         _check(lib().tecno_timing_enable(self.h, int(on)))
 
@@ -322,3 +342,24 @@ def make_train_rows(u4, target):
     rows = np.zeros((u4.shape[0], 21))
     _check(lib().tecno_make_train_rows(_dp(u4), _dp(target), u4.shape[0], _dp(rows)))
     return rows
+
+
+def harvest_stencils_1d(cells, face_values, periodic=True):
+    """Training stencils from a 1D solver state: every interface i+1/2 with
+    its 4-cell stencil (u[i-1], u[i], u[i+1], u[i+2]) and the target value on
+    both sides (face_values[i] = value at x[i+1/2]).  Stencils whose centre
+    jump is below 1e-12 of the stencil scale are dropped: the solver skips the
+    network there (|u2 - u1| < 1e-15) and their ratios carry no information."""
+    u = np.asarray(cells, dtype=np.float64)
+    n = u.shape[0]
+    idx = np.arange(n)
+    if periodic:
+        u4 = np.stack([u[(idx - 1) % n], u, u[(idx + 1) % n], u[(idx + 2) % n]], axis=1)
+        f = np.asarray(face_values, dtype=np.float64)
+    else:
+        sel = idx[1:n - 2]
+        u4 = np.stack([u[sel - 1], u[sel], u[sel + 1], u[sel + 2]], axis=1)
+        f = np.asarray(face_values, dtype=np.float64)[sel]
+    scale = np.maximum(1.0, np.abs(u4).max(axis=1))
+    keep = np.abs(u4[:, 2] - u4[:, 1]) > 1e-12 * scale
+    return u4[keep], np.stack([f[keep], f[keep]], axis=1)

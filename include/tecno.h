@@ -137,6 +137,18 @@ int tecno_set_net(tecno_ctx* ctx, int n_layers, const int32_t* widths,
                   const double* W, const double* b);
 int tecno_get_net(const tecno_ctx* ctx, int* n_layers, int32_t* widths,
                   double* W, double* b);
+/* Write the context's network in the net_file format (the trainer's
+ * save_net_weights, DSP-WENO_network_trainer.py:197-213: "Weights", per layer
+ * "Layer k" / "BEGIN" / one out-row per line / "END", then "Biases" likewise,
+ * values as %.18e). */
+int tecno_save_net(const tecno_ctx* ctx, const char* net_file);
+/* Seeded initial network of n_layers with the given widths (5 in, 5 out):
+ * weights Xavier-uniform U(+-sqrt(6/(fan_in+fan_out))) as the trainer's
+ * nn.init.xavier_uniform_ (DSP-WENO_network_trainer.py:58-60), biases
+ * U(+-1/sqrt(fan_in)) as nn.Linear's default.  The random stream is this
+ * library's own (a seed does not reproduce PyTorch's draws). */
+int tecno_init_net(tecno_ctx* ctx, int n_layers, const int32_t* widths,
+                   uint64_t seed);
 
 /* Number of interior cells held by the context (product of active n[d]). */
 int64_t tecno_num_cells(const tecno_ctx* ctx);
@@ -214,9 +226,19 @@ int tecno_train_step(tecno_ctx* ctx, const double* rows, int64_t n_rows,
  * b, as tecno_get_net. */
 int tecno_train_grad(tecno_ctx* ctx, const double* rows, int64_t n_rows,
                      int use_bf16, double* loss_out, double* grad_out);
-/* Build training rows from raw 4-cell stencils and targets:
+/* Keep n_rows training rows (n x 21, as above) on the device for
+ * tecno_train_epoch. */
+int tecno_train_set_data(tecno_ctx* ctx, const double* rows, int64_t n_rows);
+/* One pass over the device data set in mini-batches of `batch` rows taken in
+ * a seeded random order (DataLoader(shuffle=True), trainer:336-347; the last
+ * batch may be shorter), one Adam step per batch.  loss_out = mean row loss
+ * over the pass. */
+int tecno_train_epoch(tecno_ctx* ctx, int64_t batch, uint64_t seed,
+                      int use_bf16, double* loss_out);
+/* Build training rows from raw 4-cell stencils and targets on the device:
  * u4 (n x 4), target (n x 2) -> rows (n x 21), computing features and
- * vertices as Neural_Network::forward does. */
+ * vertices as Neural_Network::forward does (scaled stencil features,
+ * compute_vertices with vertex_mode 2, permutation 2). */
 int tecno_make_train_rows(const double* u4, const double* target, int64_t n,
                           double* rows);
 

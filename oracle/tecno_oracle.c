@@ -1246,3 +1246,138 @@ void tor_audit(const tor_solver* s, double out[8]) {
   out[6] = e * dV;
   out[7] = smax;
 }
+
+/* ------------------------------------------------------------------------ */
+/* trainer semantics (DSP-WENO_network_trainer.py)                          */
+/* Rows: features/vertices as compute_vertices + the forward's inputs
+ * (neural_network.cc:16-77, 135-260), i.e. what the trainer reads from its
+ * data file columns (trainer:281-333).                                      */
+void tor_train_rows(const double* u4, const double* target, int64_t n,
+                    double* rows) {
+  for (int64_t i = 0; i < n; ++i) {
+    double* r = rows + i * TOR_ROW;
+    tor_nn_features(u4 + 4 * i, r, r + 5, r + 10);
+    for (int k = 0; k < 4; ++k) r[15 + k] = u4[4 * i + k];
+    r[19] = target[2 * i];
+    r[20] = target[2 * i + 1];
+  }
+}
+
+static double bf16_round(double x) {
+  float f = (float)x;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+/* MLP.forward (trainer:63-91): Linear layers with ReLU between them, softmax,
+ * C1 = s.xv, C2 = s.yv, omega1 = 1/4 - 2 C1, omega0~ = 1/4 - 2 C2;
+ * inter_evaluation (trainer:94-106): left = (1-omega1)(u1+u2)/2 +
+ * omega1 (3u1-u0)/2, right = omega0~ (3u2-u3)/2 + (1-omega0~)(u1+u2)/2;
+ * loss (trainer:355-356) = (MSE(left,tL) + MSE(right,tR)) / 2.  The gradient
+ * is the analytic backward of exactly these expressions. */
+double tor_train_loss_grad(const tor_net* net, const double* rows, int64_t n,
+                           int bf16, tor_net* grad) {
+  const int L = net->n_layers;
+  tor_net Wq = *net;
+  if (bf16)
+    for (int l = 0; l < L; ++l) {
+      for (int i = 0; i < TOR_MAX_WIDTH * TOR_MAX_WIDTH; ++i)
+        Wq.W[l][i] = bf16_round(net->W[l][i]);
+      for (int i = 0; i < TOR_MAX_WIDTH; ++i) Wq.b[l][i] = bf16_round(net->b[l][i]);
+    }
+  if (grad) {
+    memset(grad, 0, sizeof(*grad));
+    grad->n_layers = L;
+    for (int l = 0; l <= L; ++l) grad->width[l] = net->width[l];
+  }
+  double loss = 0.0;
+  for (int64_t r = 0; r < n; ++r) {
+    const double* row = rows + r * TOR_ROW;
+    double h[TOR_MAX_LAYERS + 1][TOR_MAX_WIDTH];
+    for (int k = 0; k < 5; ++k) h[0][k] = bf16 ? bf16_round(row[k]) : row[k];
+    for (int l = 0; l < L; ++l) {
+      const int no = net->width[l + 1], ni = net->width[l];
+      for (int j = 0; j < no; ++j) {
+        double z = Wq.b[l][j];
+        for (int k = 0; k < ni; ++k) z += Wq.W[l][j * TOR_MAX_WIDTH + k] * h[l][k];
+        if (l != L - 1) z = dmax(0.0, z);
+        h[l + 1][j] = (bf16 && l != L - 1) ? bf16_round(z) : z;
+      }
+    }
+    const double* lg = h[L];
+    double mx = lg[0];
+    for (int i = 1; i < 5; ++i) mx = dmax(mx, lg[i]);
+    double s[5], sum = 0.0;
+    for (int i = 0; i < 5; ++i) {
+      s[i] = exp(lg[i] - mx);
+      sum += s[i];
+    }
+    double C1 = 0.0, C2 = 0.0;
+    for (int i = 0; i < 5; ++i) {
+      s[i] /= sum;
+      C1 += s[i] * row[5 + i];
+      C2 += s[i] * row[10 + i];
+    }
+    const double u0 = row[15], u1 = row[16], u2 = row[17], u3 = row[18];
+    const double w1 = 0.25 - 2.0 * C1, tw0 = 0.25 - 2.0 * C2;
+    const double left = (1.0 - w1) * 0.5 * (u1 + u2) + w1 * (1.5 * u1 - 0.5 * u0);
+    const double right = tw0 * (1.5 * u2 - 0.5 * u3) + (1.0 - tw0) * 0.5 * (u1 + u2);
+    const double eL = left - row[19], eR = right - row[20];
+    loss += 0.5 * (eL * eL + eR * eR);
+    if (!grad) continue;
+    /* d loss / d left = eL / n; d left / d C1 = -2 (1.5u1 - 0.5u0 - 0.5(u1+u2)) */
+    const double gC1 = (eL / (double)n) * (-2.0) * (u1 - 0.5 * u0 - 0.5 * u2);
+    const double gC2 = (eR / (double)n) * (-2.0) * (u2 - 0.5 * u3 - 0.5 * u1);
+    double gz[TOR_MAX_WIDTH] = {0};
+    for (int i = 0; i < 5; ++i)
+      gz[i] = s[i] * ((row[5 + i] - C1) * gC1 + (row[10 + i] - C2) * gC2);
+    for (int l = L - 1; l >= 0; --l) {
+      const int no = net->width[l + 1], ni = net->width[l];
+      double gh[TOR_MAX_WIDTH] = {0};
+      for (int j = 0; j < no; ++j) {
+        grad->b[l][j] += gz[j];
+        for (int k = 0; k < ni; ++k) {
+          grad->W[l][j * TOR_MAX_WIDTH + k] += gz[j] * h[l][k];
+          gh[k] += Wq.W[l][j * TOR_MAX_WIDTH + k] * gz[j];
+        }
+      }
+      if (l > 0)
+        for (int k = 0; k < ni; ++k) gz[k] = (h[l][k] > 0.0) ? gh[k] : 0.0;
+    }
+  }
+  return loss / (double)n;
+}
+
+/* torch.optim.Adam (weight_decay as L2 on the gradient, no amsgrad):
+ * g += wd p; m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
+ * p -= (lr / (1-b1^t)) m / (sqrt(v) / sqrt(1-b2^t) + eps)                  */
+static void adam_one(double* p, double g, double* m, double* v, double bc1,
+                     double bc2, double lr, double b1, double b2, double eps,
+                     double wd) {
+  g += wd * (*p);
+  *m = b1 * (*m) + (1.0 - b1) * g;
+  *v = b2 * (*v) + (1.0 - b2) * g * g;
+  *p -= (lr / bc1) * (*m) / (sqrt(*v) / sqrt(bc2) + eps);
+}
+
+void tor_adam_step(tor_net* net, const tor_net* grad, tor_net* m, tor_net* v,
+                   int64_t t, double lr, double beta1, double beta2, double eps,
+                   double weight_decay) {
+  const double bc1 = 1.0 - pow(beta1, (double)t), bc2 = 1.0 - pow(beta2, (double)t);
+  for (int l = 0; l < net->n_layers; ++l) {
+    const int no = net->width[l + 1], ni = net->width[l];
+    for (int j = 0; j < no; ++j) {
+      for (int k = 0; k < ni; ++k) {
+        const int q = j * TOR_MAX_WIDTH + k;
+        adam_one(&net->W[l][q], grad->W[l][q], &m->W[l][q], &v->W[l][q], bc1, bc2,
+                 lr, beta1, beta2, eps, weight_decay);
+      }
+      adam_one(&net->b[l][j], grad->b[l][j], &m->b[l][j], &v->b[l][j], bc1, bc2, lr,
+               beta1, beta2, eps, weight_decay);
+    }
+  }
+}

@@ -87,6 +87,25 @@ void tor_set_threads(int n);
  * out[3..5] momentum, out[6] energy, out[7] max speed */
 void tor_audit(const tor_solver* s, double out[8]);
 
+/* ---- trainer (DSP-WENO_trainer/DSP-WENO_network_trainer.py) --------------
+ * A training row holds 21 values: the 5 network inputs (tanh theta-,
+ * tanh theta+, |D0|, |D1|, |D2| of the scaled stencil), 5 x-vertices,
+ * 5 y-vertices, the 4 unscaled cells and the 2 target interface values. */
+#define TOR_ROW 21
+void tor_train_rows(const double* u4, const double* target, int64_t n,
+                    double* rows);
+/* Loss 0.5 (MSE(left, tL) + MSE(right, tR)) over n rows and its gradient
+ * (same layout as the network: W[l][out * TOR_MAX_WIDTH + in], b[l][out]).
+ * bf16 != 0 rounds the weights and every activation to bfloat16 (nearest
+ * even) before use, as a bfloat16-activation forward does. */
+double tor_train_loss_grad(const tor_net* net, const double* rows, int64_t n,
+                           int bf16, tor_net* grad);
+/* torch.optim.Adam step t (1-based) with L2 weight decay added to the
+ * gradient; m, v are the moment buffers in the network layout. */
+void tor_adam_step(tor_net* net, const tor_net* grad, tor_net* m, tor_net* v,
+                   int64_t t, double lr, double beta1, double beta2, double eps,
+                   double weight_decay);
+
 #ifdef __cplusplus
 }
 #endif

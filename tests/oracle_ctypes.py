@@ -51,6 +51,12 @@ def lib():
         L.tor_advance.argtypes = [P, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         L.tor_set_threads.argtypes = [C.c_int]
         L.tor_audit.argtypes = [P, dp]
+        L.tor_train_rows.argtypes = [dp, dp, C.c_int64, dp]
+        L.tor_train_loss_grad.argtypes = [C.POINTER(Net), dp, C.c_int64, C.c_int, C.POINTER(Net)]
+        L.tor_train_loss_grad.restype = C.c_double
+        L.tor_adam_step.argtypes = [C.POINTER(Net), C.POINTER(Net), C.POINTER(Net), C.POINTER(Net),
+                                    C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double]
         _lib = L
     return _lib
 
@@ -201,3 +207,48 @@ class OracleSolver:
         out = np.zeros(8)
         lib().tor_audit(self.h, _dp(out))
         return out
+
+
+# ---------------------------------------------------------------- trainer
+def train_rows(u4, target):
+    u4 = np.ascontiguousarray(u4, dtype=np.float64).reshape(-1, 4)
+    target = np.ascontiguousarray(target, dtype=np.float64).reshape(-1, 2)
+    rows = np.zeros((u4.shape[0], 21))
+    lib().tor_train_rows(_dp(u4), _dp(target), u4.shape[0], _dp(rows))
+    return rows
+
+
+def grad_arrays(g):
+    """flatten a gradient held in a Net: W (layer, out, in) then b"""
+    _, W, b = net_arrays(g)
+    return np.concatenate([W, b])
+
+
+def train_loss_grad(net, rows, bf16=False):
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    g = Net()
+    loss = lib().tor_train_loss_grad(C.byref(net), _dp(rows), rows.shape[0], int(bf16), C.byref(g))
+    return loss, grad_arrays(g)
+
+
+def train_loss(net, rows, bf16=False):
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    return lib().tor_train_loss_grad(C.byref(net), _dp(rows), rows.shape[0], int(bf16), None)
+
+
+def adam_step(net, rows, m, v, t, lr=1e-3, beta1=0.5, beta2=0.9, eps=1e-8, wd=1e-5):
+    """one trainer step on the oracle: gradient of rows, then Adam; m, v are
+    Nets holding the moments (zero-initialised by the caller)"""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    g = Net()
+    loss = lib().tor_train_loss_grad(C.byref(net), _dp(rows), rows.shape[0], 0, C.byref(g))
+    lib().tor_adam_step(C.byref(net), C.byref(g), C.byref(m), C.byref(v), t, lr, beta1, beta2, eps, wd)
+    return loss
+
+
+def zero_like(net):
+    z = Net()
+    z.n_layers = net.n_layers
+    for i in range(net.n_layers + 1):
+        z.width[i] = net.width[i]
+    return z

@@ -59,6 +59,7 @@ def time_steps(s, n, steps):
     """K steps bracketed by synchronisation; stage-kernel time from HIP events
     recorded on the solver's stream."""
     s.sync()
+    fb0 = s.audit().n_sign_fallback
     s.timing(True)
     t0 = time.perf_counter()
     s.run_steps(steps, final_time=1.0, t_stamps=1)
@@ -72,7 +73,7 @@ def time_steps(s, n, steps):
         raise RuntimeError("non-finite state")
     return dict(seconds=el, cell_updates_per_s=n * n * steps / el, stage_ms=stage_ms,
                 n_stages=nst, ms_per_step=1e3 * el / steps, time=t, iters=it,
-                sign_fallbacks=a.n_sign_fallback)
+                sign_fallbacks=a.n_sign_fallback - fb0)
 
 
 KH_PARAM = """grid
@@ -209,6 +210,17 @@ def cpu_baseline(n_sample=512, steps=2):
             "sample": f"KH {n_sample}x{n_sample}x2, {steps} step(s), sp_weno_dl, oracle C port, 1 core"}
 
 
+VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: CUs x SIMDs x clock / 4 cycles
+
+
+def load_profile(kind, variant):
+    p = os.path.join(ROOT, "profiles", f"{kind}_{variant}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
 def load_traffic(variant):
     p = os.path.join(ROOT, "profiles", f"traffic_{variant}.json")
     if os.path.exists(p):
@@ -224,7 +236,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="", help="headline | fp64 (profiling runs)")
+    ap.add_argument("--only", default="", help="headline | fp64 | fp64dsp (profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one plain 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -245,7 +257,7 @@ def main():
     import tecno
 
     n = args.n
-    head_prec, head_rec, head_mlp = tecno.FP32, "sp_weno_dl", tecno.MLP_FP32
+    head_prec, head_rec, head_mlp = tecno.FP32, "sp_weno_dl", tecno.MLP_BF16
     res = {}
     # both paths are timed from the same developed state: the headline solver
     # advances the KH initial state by `warmup` steps (untimed), its state is
@@ -259,9 +271,16 @@ def main():
     t_spun, _ = sh.time()
     if args.only in ("", "headline"):
         res["headline"] = time_steps(sh, n, args.steps)
-        print(f"# headline (mixed FP64/FP32, sp_weno_dl) from t={t_spun:.4f}: {res['headline']}",
-              file=sys.stderr, flush=True)
+        print(f"# headline (mixed FP64/FP32, sp_weno_dl, bf16 MFMA network) from t={t_spun:.4f}: "
+              f"{res['headline']}", file=sys.stderr, flush=True)
     sh.close()
+    if args.only in ("", "fp64dsp"):
+        sd = make_solver(tecno, n, tecno.FP64, "sp_weno_dl", tecno.MLP_FP64)
+        sd.set_state(spun)
+        sd.run_steps(1, final_time=1.0, t_stamps=1)
+        res["fp64_dsp"] = time_steps(sd, n, args.steps)
+        sd.close()
+        print(f"# FP64 sp_weno_dl: {res['fp64_dsp']}", file=sys.stderr, flush=True)
     if args.only in ("", "fp64"):
         sb = make_solver(tecno, n, tecno.FP64, "sp_weno", tecno.MLP_FP64)
         sb.set_state(spun)
@@ -291,22 +310,35 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp64 state + fp64 entropy-conservative flux, fp32 dissipation/reconstruction/MLP",
+        "dtype": "fp64 state + fp64 entropy-conservative flux, fp32 dissipation/reconstruction, "
+                 "bf16 MLP (fp32 accumulation)",
         "data": "synthetic (seeded KH initial state; shipped DSP-WENO weights)",
         "config": {"workload": f"2D Euler Kelvin-Helmholtz {n}x{n} periodic, kepes_tecno4_roe + "
                                f"sp_weno_dl (DSP-WENO), SSPRK3 CFL 0.5",
                    "cells": n * n, "flux": "kepes_tecno4_roe", "reconstruct": "sp_weno_dl",
-                   "precision": "fp64 state, cell records and entropy-conservative flux; fp32 "
-                                "eigen-dissipation, jump-form reconstruction and SIMT MLP",
+                   "precision": "fp64 state, cell records, entropy-conservative flux and "
+                                "eigen-projection; fp32 jump-form reconstruction and dissipation; "
+                                "DSP-WENO network as bf16 MFMA 32x32x16 with fp32 accumulation",
                    "timed_from_time": t_spun},
         "fp64_spweno_cell_updates_per_s": b["cell_updates_per_s"],
         "speedup_vs_fp64_spweno": h["cell_updates_per_s"] / b["cell_updates_per_s"],
+        "fp64_dsp_cell_updates_per_s": res["fp64_dsp"]["cell_updates_per_s"],
+        "speedup_vs_fp64_dsp": h["cell_updates_per_s"] / res["fp64_dsp"]["cell_updates_per_s"],
+        "sign_fallbacks_per_step": h["sign_fallbacks"] / max(args.steps, 1),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "stage_kernel<Euler2Mixed<DSP,SIMT>>",
+                     "kernel": "stage_kernel<Euler2Mixed<DSP,BF16>>",
                      "stage_ms": h["stage_ms"], "bytes_per_launch_alg": bytes_per_launch},
     }
+    valu = load_profile("valu", "headline")
+    if valu:
+        # instructions per stage dispatch from the committed counter profile,
+        # divided by this run's measured stage time
+        rate = valu["valu_insts_per_dispatch"] / (h["stage_ms"] * 1e-3)
+        out["valu_roofline"] = {"achieved": rate, "peak": VALU_PEAK,
+                                "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK,
+                                "insts_per_launch": valu["valu_insts_per_dispatch"]}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

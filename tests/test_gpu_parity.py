@@ -367,17 +367,20 @@ def _vortex(n, t=0.0):
                      beta / (2 * math.pi) * Xs * np.exp(0.5 * (1 - r2)), T ** (g / (g - 1))]).reshape(4, -1)
 
 
-@pytest.mark.parametrize("rec", ["sp_weno", "sp_weno_dl"])
-def test_vortex_l1_and_order_mixed_matches_fp64(rec):
+@pytest.mark.parametrize("rec,mlp", [("sp_weno", MLP_FP32), ("sp_weno_dl", MLP_FP32),
+                                     ("sp_weno_dl", MLP_BF16), ("sp_weno_dl", MLP_FP16)])
+def test_vortex_l1_and_order_mixed_matches_fp64(rec, mlp):
     """Isentropic vortex (strength 5, periodic [-5,5]^2, t = 1): the mixed
-    path's L1(rho) error is within 2% of the FP64 path's on every mesh and the
-    observed convergence orders agree to 0.05."""
+    path's L1(rho) error (FP32 SIMT or bf16/fp16 matrix-core network) is within
+    2% of the FP64 path's on every mesh and the observed convergence orders
+    agree to 0.05."""
     errs = {FP64: [], FP32: []}
     ns = [64, 128, 256]
     for prec in (FP64, FP32):
         for n in ns:
             s = solver_for(make_cfg(n=(n, n, 2), dims=2, lo=(-5, -5, 0), hi=(5, 5, 1), recon=rec,
-                                    bc="periodic", precision=prec, sign_fallback=int(prec == FP32)),
+                                    bc="periodic", precision=prec, sign_fallback=int(prec == FP32),
+                                    mlp_dtype=mlp if prec == FP32 else MLP_FP64),
                            rec == "sp_weno_dl")
             s.set_state(_vortex(n))
             it, t = s.advance(1.0, 1)

@@ -13,6 +13,7 @@
 
 #include "../../include/tecno.h"
 #include "fv_launch.hpp"
+#include "ensemble.hpp"
 #include "mlp.hpp"
 
 namespace tecno {
@@ -979,4 +980,356 @@ int tecno_init_net(tecno_ctx* c, int nl, const int32_t* widths, uint64_t seed) {
   return tecno_set_net(c, nl, widths, W.data(), b.data());
 }
 
+// ------------------------------------------------------------------ ensembles
 }  // extern "C"
+
+namespace {
+struct EnsGroupH {
+  bool fp32 = false;
+  int dims = 1;
+  int variant = 0;
+  std::vector<int> members;
+  void* dprobs = nullptr;
+  EnsWork* dwork = nullptr;
+  int nwork = 0;
+  int grid = 0;
+  double tuned_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+}  // namespace
+
+struct tecno_ens {
+  std::vector<tecno_ctx*> m;
+  std::vector<EnsGroupH> g;
+  EnsCtl* dctl = nullptr;
+  int* dcount = nullptr;
+  hipStream_t s = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
+  int n_cu = 0;
+};
+
+static int ens_nvariants(int dims) { return dims == 1 ? ENS_NV1 : ENS_NV2; }
+
+static void ens_drop_graph(tecno_ens* e) {
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->gexec = nullptr;
+  e->graph = nullptr;
+  e->graph_valid = false;
+}
+
+// work list of a group for its current tile shape
+static int ens_build_work(tecno_ens* e, EnsGroupH& G) {
+  const int TX = ens_tx(G.dims, G.variant), TY = ens_ty(G.dims, G.variant);
+  std::vector<EnsWork> w;
+  for (int k = 0; k < (int)G.members.size(); ++k) {
+    const tecno_ctx* c = e->m[G.members[k]];
+    const int tx = (c->n[0] + TX - 1) / TX;
+    const int ty = G.dims == 2 ? (c->n[1] + TY - 1) / TY : 1;
+    for (int by = 0; by < ty; ++by)
+      for (int bx = 0; bx < tx; ++bx) w.push_back(EnsWork{k, bx, by, 0});
+  }
+  if (G.dwork) HIPCHK(hipFree(G.dwork));
+  G.dwork = nullptr;
+  G.nwork = (int)w.size();
+  HIPCHK(hipMalloc(&G.dwork, w.size() * sizeof(EnsWork)));
+  HIPCHK(hipMemcpy(G.dwork, w.data(), w.size() * sizeof(EnsWork), hipMemcpyHostToDevice));
+  const int resident = ens_grid(G.fp32, G.dims, G.variant, e->n_cu);
+  G.grid = std::max(1, std::min(G.nwork, resident));
+  ens_drop_graph(e);
+  return 0;
+}
+
+template <typename T>
+static int ens_upload_probs_t(tecno_ens* e, EnsGroupH& G, const Phys<T> tecno_ctx::*pm) {
+  std::vector<EnsProb<T>> h(G.members.size());
+  for (size_t k = 0; k < G.members.size(); ++k) {
+    tecno_ctx* c = e->m[G.members[k]];
+    h[k].P = c->*pm;
+    h[k].g = c->L.grid;
+    h[k].ctl = c->ctl;
+    for (int q = 0; q < 3; ++q) h[k].U[q] = c->U[q];
+  }
+  if (!G.dprobs) HIPCHK(hipMalloc(&G.dprobs, h.size() * sizeof(EnsProb<T>)));
+  HIPCHK(hipMemcpy(G.dprobs, h.data(), h.size() * sizeof(EnsProb<T>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int ens_upload(tecno_ens* e) {
+  std::vector<EnsCtl> hc(e->m.size());
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    tecno_ctx* c = e->m[i];
+    const bool linadv = c->cfg.model == TECNO_MODEL_LINADV;
+    hc[i] = EnsCtl{c->ctl, c->hmin, c->cfg.cfl, c->P64.vel_norm, linadv ? 1 : 0, 0};
+  }
+  if (!e->dctl) HIPCHK(hipMalloc(&e->dctl, hc.size() * sizeof(EnsCtl)));
+  HIPCHK(hipMemcpy(e->dctl, hc.data(), hc.size() * sizeof(EnsCtl), hipMemcpyHostToDevice));
+  for (auto& G : e->g) {
+    const int rc = G.fp32 ? ens_upload_probs_t<float>(e, G, &tecno_ctx::P32)
+                          : ens_upload_probs_t<double>(e, G, &tecno_ctx::P64);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// one ensemble step on stream s: dt of every member, 3 stages per group, end
+static hipError_t ens_step(tecno_ens* e, int only_group) {
+  const int n = (int)e->m.size();
+  hipError_t r = ens_control(e->s, e->dctl, n, 0);
+  for (int st = 0; st < 3 && r == hipSuccess; ++st)
+    for (int gi = 0; gi < (int)e->g.size() && r == hipSuccess; ++gi) {
+      if (only_group >= 0 && gi != only_group) continue;
+      const EnsGroupH& G = e->g[gi];
+      r = ens_stage(e->s, G.fp32, G.dims, G.variant, G.dprobs, G.dwork, G.nwork, st, G.grid);
+    }
+  if (r == hipSuccess) r = ens_control(e->s, e->dctl, n, 1);
+  return r;
+}
+
+static int ens_capture(tecno_ens* e) {
+  ens_drop_graph(e);
+  HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeGlobal));
+  const hipError_t r = ens_step(e, -1);
+  hipGraph_t graph = nullptr;
+  const hipError_t r2 = hipStreamEndCapture(e->s, &graph);
+  HIPCHK(r);
+  HIPCHK(r2);
+  e->graph = graph;
+  HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+  e->graph_valid = true;
+  return 0;
+}
+
+static int ens_sync_members(tecno_ens* e) {
+  for (tecno_ctx* c : e->m) HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" {
+
+int tecno_ens_create(const tecno_cfg* cfgs, int n, tecno_ens** out) {
+  if (!cfgs || !out || n < 1) return fail(TECNO_EINVAL, "null argument or empty ensemble");
+  *out = nullptr;
+  for (int i = 0; i < n; ++i) {
+    const tecno_cfg& g = cfgs[i];
+    if (g.system != TECNO_SYS_SCALAR || g.dims > 2)
+      return fail(TECNO_EINVAL, "ensembles hold 1D or 2D scalar problems");
+    if (g.time_scheme != TECNO_TS_SSPRK3) return fail(TECNO_EINVAL, "ensembles step with SSPRK3");
+    if (g.recon == TECNO_REC_ENO && g.eno_level == 3)
+      return fail(TECNO_EINVAL, "ensembles use reconstructions with 2 ghost cells");
+  }
+  tecno_ens* e = new tecno_ens();
+  auto bail = [&](int rc) {
+    tecno_ens_destroy(e);
+    return rc;
+  };
+  for (int i = 0; i < n; ++i) {
+    tecno_ctx* c = nullptr;
+    if (int rc = tecno_create(&cfgs[i], &c)) return bail(rc);
+    e->m.push_back(c);
+    if (c->G != 2) return bail(fail(TECNO_EINVAL, "ensembles use reconstructions with 2 ghost cells"));
+    const bool fp32 = cfgs[i].precision == TECNO_FP32;
+    const int dims = cfgs[i].dims;
+    int gi = -1;
+    for (int q = 0; q < (int)e->g.size(); ++q)
+      if (e->g[q].fp32 == fp32 && e->g[q].dims == dims) gi = q;
+    if (gi < 0) {
+      e->g.emplace_back();
+      gi = (int)e->g.size() - 1;
+      e->g[gi].fp32 = fp32;
+      e->g[gi].dims = dims;
+    }
+    e->g[gi].members.push_back(i);
+  }
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return bail(fail(TECNO_EHIP, "hipGetDeviceProperties failed"));
+  e->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(TECNO_EHIP, "hipStreamCreate failed"));
+  if (hipMalloc(&e->dcount, sizeof(int)) != hipSuccess)
+    return bail(fail(TECNO_ENOMEM, "hipMalloc failed"));
+  for (auto& G : e->g)
+    if (int rc = ens_build_work(e, G)) return bail(rc);
+  *out = e;
+  return 0;
+}
+
+void tecno_ens_destroy(tecno_ens* e) {
+  if (!e) return;
+  if (e->s) (void)hipStreamSynchronize(e->s);
+  ens_drop_graph(e);
+  for (auto& G : e->g) {
+    if (G.dprobs) (void)hipFree(G.dprobs);
+    if (G.dwork) (void)hipFree(G.dwork);
+  }
+  if (e->dctl) (void)hipFree(e->dctl);
+  if (e->dcount) (void)hipFree(e->dcount);
+  for (tecno_ctx* c : e->m) tecno_destroy(c);
+  if (e->s) (void)hipStreamDestroy(e->s);
+  delete e;
+}
+
+int tecno_ens_size(const tecno_ens* e) { return e ? (int)e->m.size() : 0; }
+
+tecno_ctx* tecno_ens_member(tecno_ens* e, int i) {
+  if (!e || i < 0 || i >= (int)e->m.size()) return nullptr;
+  return e->m[i];
+}
+
+int tecno_ens_num_groups(const tecno_ens* e) { return e ? (int)e->g.size() : 0; }
+
+int tecno_ens_group_info(const tecno_ens* e, int gi, tecno_ens_group* out) {
+  if (!e || !out || gi < 0 || gi >= (int)e->g.size()) return fail(TECNO_EINVAL, "bad group");
+  const EnsGroupH& G = e->g[gi];
+  std::memset(out, 0, sizeof(*out));
+  out->precision = G.fp32 ? TECNO_FP32 : TECNO_FP64;
+  out->dims = G.dims;
+  out->n_members = (int)G.members.size();
+  out->n_tiles = G.nwork;
+  out->grid = G.grid;
+  out->tile_x = ens_tx(G.dims, G.variant);
+  out->tile_y = ens_ty(G.dims, G.variant);
+  out->n_candidates = ens_nvariants(G.dims);
+  for (int v = 0; v < out->n_candidates; ++v) {
+    out->cand_x[v] = ens_tx(G.dims, v);
+    out->cand_y[v] = ens_ty(G.dims, v);
+    out->tuned_ms[v] = G.tuned_ms[v];
+  }
+  return 0;
+}
+
+int tecno_ens_advance(tecno_ens* e, const double* final_times, int t_stamps, int max_iter,
+                      int use_graph, int64_t* cell_updates) {
+  if (!e || !final_times) return fail(TECNO_EINVAL, "null argument");
+  if (t_stamps < 1) return fail(TECNO_EINVAL, "t_stamps must be >= 1");
+  std::vector<int> it0(e->m.size());
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    tecno_ctx* c = e->m[i];
+    if (!c->state_set) return fail(TECNO_ESTATE, "tecno_set_state on every member first");
+    if (int rc = need_net(c)) return rc;
+    if (int rc = set_schedule(c, final_times[i], t_stamps, max_iter)) return rc;
+  }
+  if (int rc = ens_sync_members(e)) return rc;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    Ctl h{};
+    HIPCHK(hipMemcpy(&h, e->m[i]->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    it0[i] = h.iter;
+  }
+  if (int rc = ens_upload(e)) return rc;
+  if (use_graph && !e->graph_valid)
+    if (int rc = ens_capture(e)) return rc;
+  int active = 1;
+  while (active) {
+    for (int r = 0; r < 16; ++r) {
+      if (use_graph)
+        HIPCHK(hipGraphLaunch(e->gexec, e->s));
+      else
+        HIPCHK(ens_step(e, -1));
+    }
+    HIPCHK(ens_count_active(e->s, e->dctl, (int)e->m.size(), e->dcount));
+    HIPCHK(hipMemcpyAsync(&active, e->dcount, sizeof(int), hipMemcpyDeviceToHost, e->s));
+    HIPCHK(hipStreamSynchronize(e->s));
+  }
+  int64_t cu = 0;
+  bool bad = false;
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    Ctl h{};
+    HIPCHK(hipMemcpy(&h, e->m[i]->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    cu += (int64_t)(h.iter - it0[i]) * e->m[i]->N;
+    bad = bad || h.nbad;
+  }
+  if (cell_updates) *cell_updates = cu;
+  if (bad) return fail(TECNO_ENONFINITE, "non-finite state in an ensemble member");
+  return 0;
+}
+
+int tecno_ens_autotune(tecno_ens* e, int steps) {
+  if (!e || steps < 1) return fail(TECNO_EINVAL, "bad argument");
+  for (tecno_ctx* c : e->m)
+    if (!c->state_set) return fail(TECNO_ESTATE, "tecno_set_state on every member first");
+  if (int rc = ens_sync_members(e)) return rc;
+  // save every member's state and control block
+  std::vector<void*> save(e->m.size(), nullptr);
+  std::vector<Ctl> ctl(e->m.size());
+  auto cleanup = [&]() {
+    for (void* p : save)
+      if (p) (void)hipFree(p);
+  };
+  for (size_t i = 0; i < e->m.size(); ++i) {
+    tecno_ctx* c = e->m[i];
+    const size_t bytes = (size_t)c->N * c->nvar * c->esize;
+    if (hipMalloc(&save[i], bytes) != hipSuccess) {
+      cleanup();
+      return fail(TECNO_ENOMEM, "autotune: no memory for the state copies");
+    }
+    HIPCHK(hipMemcpy(save[i], c->U[0], bytes, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(&ctl[i], c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  }
+  auto restore = [&]() -> int {
+    for (size_t i = 0; i < e->m.size(); ++i) {
+      tecno_ctx* c = e->m[i];
+      HIPCHK(hipMemcpy(c->U[0], save[i], (size_t)c->N * c->nvar * c->esize,
+                       hipMemcpyDeviceToDevice));
+      Ctl h = ctl[i];
+      h.final_time = INFINITY;  // keep every member stepping while timed
+      h.t_save = INFINITY;
+      h.t_int = INFINITY;
+      h.max_iter = 0x7fffffff;
+      HIPCHK(hipMemcpy(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice));
+    }
+    return 0;
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = ens_upload(e);
+  for (int gi = 0; gi < (int)e->g.size() && !rc; ++gi) {
+    EnsGroupH& G = e->g[gi];
+    double best = 1e300;
+    int bestv = 0;
+    for (int v = 0; v < ens_nvariants(G.dims) && !rc; ++v) {
+      G.variant = v;
+      if ((rc = ens_build_work(e, G)) || (rc = restore())) break;
+      if (ens_step(e, gi) != hipSuccess) {  // warm-up step, not timed
+        rc = fail(TECNO_EHIP, "autotune launch failed");
+        break;
+      }
+      (void)hipEventRecord(e0, e->s);
+      for (int k = 0; k < steps; ++k) (void)ens_step(e, gi);
+      (void)hipEventRecord(e1, e->s);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        rc = fail(TECNO_EHIP, "autotune: kernel failed");
+        break;
+      }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      G.tuned_ms[v] = ms / steps;
+      if (ms < best) {
+        best = ms;
+        bestv = v;
+      }
+    }
+    if (!rc) {
+      G.variant = bestv;
+      rc = ens_build_work(e, G);
+    }
+  }
+  // put the states and control blocks back as they were
+  for (size_t i = 0; i < e->m.size() && !rc; ++i) {
+    tecno_ctx* c = e->m[i];
+    if (hipMemcpy(c->U[0], save[i], (size_t)c->N * c->nvar * c->esize,
+                  hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(c->ctl, &ctl[i], sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(TECNO_EHIP, "autotune: restoring the state failed");
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  cleanup();
+  return rc;
+}
+
+}  // extern "C"
+

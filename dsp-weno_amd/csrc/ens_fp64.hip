@@ -1,0 +1,77 @@
+// ens_fp64.hip — FP64 ensemble stage kernels, the ensemble control kernels
+// and the dispatch (built with FP contraction off).
+#include "ens_kernels.hpp"
+
+namespace tecno {
+
+template hipError_t ens_launch<double>(hipStream_t, int, int, const void*, const EnsWork*, int, int,
+                                       int);
+template int ens_grid_of<double>(int, int, int);
+// the FP32 kernels are built in ens_fp32.hip
+extern template hipError_t ens_launch<float>(hipStream_t, int, int, const void*, const EnsWork*,
+                                             int, int, int);
+extern template int ens_grid_of<float>(int, int, int);
+
+__global__ void k_ens_control(const EnsCtl* cs, int n, int finish) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const EnsCtl e = cs[i];
+  Ctl* c = e.ctl;
+  if (finish) {  // k_finish
+    if (!c->active) return;
+    c->time += c->dt;
+    c->iter += 1;
+    if (c->time == c->t_save) c->t_save += c->t_int;
+    return;
+  }
+  // k_dt: FV::compute_dt (enflo_scalar/src_scalar/fv.cc:286-311) and the
+  // output-stamp clamp
+  if (!(c->time < c->final_time && c->iter < c->max_iter)) {
+    c->active = 0;
+    c->dt = 0.0;
+    return;
+  }
+  c->active = 1;
+  const double smax = e.const_speed ? e.speed : __longlong_as_double((long long)c->smax_bits);
+  double dt = 1.0e20;
+  const double q = e.hmin / smax;
+  dt = (q < dt) ? q : dt;
+  dt *= e.cfl;
+  if (c->time + dt > c->t_save) dt = c->t_save - c->time;
+  c->dt = dt;
+  c->smax_bits = 0ull;
+}
+
+__global__ void k_ens_count(const EnsCtl* cs, int n, int* out) {
+  // one wave: count problems that will still advance
+  int cnt = 0;
+  for (int i = threadIdx.x; i < n; i += 64) {
+    const Ctl* c = cs[i].ctl;
+    cnt += (c->time < c->final_time && c->iter < c->max_iter) ? 1 : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if (threadIdx.x == 0) *out = cnt;
+}
+
+hipError_t ens_control(hipStream_t s, const EnsCtl* c, int n, int finish) {
+  hipLaunchKernelGGL(k_ens_control, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, c, n, finish);
+  return hipGetLastError();
+}
+
+hipError_t ens_count_active(hipStream_t s, const EnsCtl* c, int n, int* out) {
+  hipLaunchKernelGGL(k_ens_count, dim3(1), dim3(64), 0, s, c, n, out);
+  return hipGetLastError();
+}
+
+hipError_t ens_stage(hipStream_t s, bool fp32, int dims, int variant, const void* probs,
+                     const EnsWork* work, int nwork, int stage, int grid) {
+  if (nwork <= 0) return hipSuccess;
+  return fp32 ? ens_launch<float>(s, dims, variant, probs, work, nwork, stage, grid)
+              : ens_launch<double>(s, dims, variant, probs, work, nwork, stage, grid);
+}
+
+int ens_grid(bool fp32, int dims, int variant, int n_cu) {
+  return fp32 ? ens_grid_of<float>(dims, variant, n_cu) : ens_grid_of<double>(dims, variant, n_cu);
+}
+
+}  // namespace tecno

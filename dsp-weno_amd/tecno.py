@@ -101,6 +101,13 @@ def make_cfg(*, system="euler", flux="kepes_tecno4_roe", recon="sp_weno", n=(32,
     return c
 
 
+class EnsGroup(C.Structure):
+    _fields_ = [("precision", C.c_int), ("dims", C.c_int), ("n_members", C.c_int),
+                ("n_tiles", C.c_int), ("grid", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int),
+                ("n_candidates", C.c_int), ("cand_x", C.c_int * 8), ("cand_y", C.c_int * 8),
+                ("tuned_ms", C.c_double * 8)]
+
+
 class Audit(C.Structure):
     _fields_ = [("total_entropy", C.c_double), ("kinetic_energy", C.c_double),
                 ("mass", C.c_double), ("momentum", C.c_double * 3), ("energy", C.c_double),
@@ -154,6 +161,16 @@ def lib():
         L.tecno_train_epoch.argtypes = [P, I64, C.c_uint64, I32, dp]
         L.tecno_save_net.argtypes = [P, C.c_char_p]
         L.tecno_init_net.argtypes = [P, I32, C.POINTER(C.c_int32), C.c_uint64]
+        L.tecno_ens_create.argtypes = [C.POINTER(Cfg), I32, C.POINTER(P)]
+        L.tecno_ens_destroy.argtypes = [P]
+        L.tecno_ens_destroy.restype = None
+        L.tecno_ens_size.argtypes = [P]
+        L.tecno_ens_member.argtypes = [P, I32]
+        L.tecno_ens_member.restype = P
+        L.tecno_ens_advance.argtypes = [P, dp, I32, I32, I32, C.POINTER(C.c_int64)]
+        L.tecno_ens_autotune.argtypes = [P, I32]
+        L.tecno_ens_num_groups.argtypes = [P]
+        L.tecno_ens_group_info.argtypes = [P, I32, C.POINTER(EnsGroup)]
         L.tecno_timing_enable.argtypes = [P, I32]
         L.tecno_timing_get.argtypes = [P, dp, C.POINTER(C.c_int64)]
         L.tecno_last_error.restype = C.c_char_p
@@ -178,15 +195,19 @@ def _check(rc):
 class Solver:
     """A context holding one grid in HBM (tecno_create ... tecno_destroy)."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, _handle=None):
         self.cfg = cfg
         self.h = C.c_void_p()
-        _check(lib().tecno_create(C.byref(cfg), C.byref(self.h)))
+        self._owned = _handle is None
+        if _handle is None:
+            _check(lib().tecno_create(C.byref(cfg), C.byref(self.h)))
+        else:
+            self.h = C.c_void_p(_handle)
 
     def close(self):
-        if self.h:
+        if self.h and self._owned:
             lib().tecno_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -363,3 +384,54 @@ def harvest_stencils_1d(cells, face_values, periodic=True):
     scale = np.maximum(1.0, np.abs(u4).max(axis=1))
     keep = np.abs(u4[:, 2] - u4[:, 1]) > 1e-12 * scale
     return u4[keep], np.stack([f[keep], f[keep]], axis=1)
+
+
+class Ensemble:
+    """Independent scalar problems advanced together (tecno_ens_*): one
+    persistent launch per precision x dimension group and stage, one
+    hipGraph per ensemble step."""
+
+    def __init__(self, cfgs):
+        arr = (Cfg * len(cfgs))(*cfgs)
+        self.h = C.c_void_p()
+        _check(lib().tecno_ens_create(arr, len(cfgs), C.byref(self.h)))
+        self.members = [Solver(cfgs[i], _handle=lib().tecno_ens_member(self.h, i))
+                        for i in range(len(cfgs))]
+
+    def close(self):
+        if self.h:
+            for m in self.members:
+                m.close()
+            lib().tecno_ens_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.members)
+
+    def advance(self, final_times, t_stamps=1, max_iter=2 ** 31 - 1, use_graph=True):
+        ft = np.ascontiguousarray(final_times, dtype=np.float64)
+        cu = C.c_int64()
+        _check(lib().tecno_ens_advance(self.h, _dp(ft), t_stamps, max_iter, int(use_graph),
+                                       C.byref(cu)))
+        return cu.value
+
+    def autotune(self, steps=3):
+        _check(lib().tecno_ens_autotune(self.h, steps))
+
+    def groups(self):
+        out = []
+        for g in range(lib().tecno_ens_num_groups(self.h)):
+            info = EnsGroup()
+            _check(lib().tecno_ens_group_info(self.h, g, C.byref(info)))
+            out.append(dict(precision="fp32" if info.precision == FP32 else "fp64", dims=info.dims,
+                            members=info.n_members, tiles=info.n_tiles, grid=info.grid,
+                            tile=(info.tile_x, info.tile_y),
+                            tuned_ms={(info.cand_x[v], info.cand_y[v]): info.tuned_ms[v]
+                                      for v in range(info.n_candidates)}))
+        return out

@@ -248,6 +248,46 @@ int tecno_make_train_rows(const double* u4, const double* target, int64_t n,
 int tecno_timing_enable(tecno_ctx* ctx, int on);
 int tecno_timing_get(tecno_ctx* ctx, double* stage_ms_avg, int64_t* n_stages);
 
+/* ---- batched ensembles of scalar problems -------------------------------
+ * No counterpart in the reference, which runs its meshes one after the other
+ * (FV::run's OOC loop, enflo_scalar/src_scalar/fv.cc:522-607): independent
+ * scalar problems (1D or 2D, any sizes, FP64 or FP32, SSPRK3, a reconstruction
+ * with 2 ghost cells) advance together.  Members of one precision and
+ * dimension share one persistent launch per stage that strides over all
+ * their tiles; one ensemble step (dt of every member, 3 stages, end of step)
+ * is captured once as a hipGraph and replayed.  Each member keeps its own dt,
+ * time and stamps; a member that has reached its end time skips its tiles.
+ * Members are ordinary contexts for set_state / get_state / l1_error /
+ * audit / get_time. */
+typedef struct tecno_ens tecno_ens;
+int tecno_ens_create(const tecno_cfg* cfgs, int n, tecno_ens** out);
+void tecno_ens_destroy(tecno_ens* ens);
+int tecno_ens_size(const tecno_ens* ens);
+tecno_ctx* tecno_ens_member(tecno_ens* ens, int i);
+/* Advance every member i to final_times[i] (t_stamps output stamps each,
+ * at most max_iter steps).  use_graph = 0 launches the kernels one by one
+ * instead of replaying the captured graph.  cell_updates_out = sum over
+ * members of steps x cells. */
+int tecno_ens_advance(tecno_ens* ens, const double* final_times, int t_stamps,
+                      int max_iter, int use_graph, int64_t* cell_updates_out);
+/* Autotuner: for every group (precision x dimension), time `steps` ensemble
+ * steps with each candidate tile shape on the members' current states (which
+ * are restored afterwards) and keep the fastest. */
+int tecno_ens_autotune(tecno_ens* ens, int steps);
+typedef struct tecno_ens_group {
+  int precision;        /* TECNO_FP64 / TECNO_FP32 */
+  int dims;
+  int n_members;
+  int n_tiles;          /* work items of one stage                          */
+  int grid;             /* persistent workgroups                            */
+  int tile_x, tile_y;   /* chosen tile shape                                */
+  int n_candidates;
+  int cand_x[8], cand_y[8];
+  double tuned_ms[8];   /* autotuner time per candidate (0 = not timed)     */
+} tecno_ens_group;
+int tecno_ens_num_groups(const tecno_ens* ens);
+int tecno_ens_group_info(const tecno_ens* ens, int g, tecno_ens_group* out);
+
 const char* tecno_last_error(void);
 const char* tecno_version(void);
 

@@ -213,6 +213,129 @@ def cpu_baseline(n_sample=512, steps=2):
 VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: CUs x SIMDs x clock / 4 cycles
 
 
+# ---------------------------------------------------------------- ensemble
+ENS_1D_N = (100, 200, 400, 600, 800, 1000)
+ENS_2D_N = (64, 96, 128, 192, 256, 384, 512, 768, 1024)
+ENS_T1D, ENS_T2D = 1.5, 0.5
+
+
+def burgers_sine_exact(x, t):
+    """entropy solution of u_t + (u^2/2)_x = 0, u(x,0) = sin x on [-pi, pi]
+    periodic, t > 1: stationary shocks at x = +-pi; inside, u = sin(xi) with
+    xi + t sin(xi) = |x| on the increasing branch xi in [0, arccos(-1/t)]
+    (bisection), odd in x"""
+    ax = np.abs(x)
+    hi_ = np.arccos(-1.0 / t) if t > 1 else np.pi
+    lo = np.zeros_like(ax)
+    hi = np.full_like(ax, hi_)
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        f = mid + t * np.sin(mid) - ax
+        lo = np.where(f < 0, mid, lo)
+        hi = np.where(f < 0, hi, mid)
+    return np.sign(x) * np.sin(0.5 * (lo + hi))
+
+
+def ens_ic2d(n, seed=4):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, 4, size=(8, 2))
+    ph = rng.uniform(0, 2 * np.pi, 8)
+    amp = rng.normal(size=8)
+    amp *= 0.05 / np.abs(amp).sum()
+
+    def f(X, Y):
+        u = np.sin(np.pi * X) * np.sin(np.pi * Y)
+        for q in range(8):
+            u = u + amp[q] * np.sin(np.pi * (k[q, 0] * X + k[q, 1] * Y) + ph[q])
+        return u
+    h = 2.0 / n
+    x = -1 + (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(x, x)
+    return f, X, Y
+
+
+def ensemble_bench(tecno, use_graph=True):
+    """Config 4: 48 periodic scalar problems (6 x 1D Burgers sin x to t=1.5,
+    9 x 2D Burgers and 9 x 2D linear advection (a = (1,1)) of
+    sin(pi x) sin(pi y) + 8 seeded Fourier modes of total amplitude 0.05 to
+    t = 0.5, each at FP64 and FP32; bur_tecno4 / lin_tecno4 + sp_weno, SSPRK3
+    CFL 0.5) in one ensemble: autotuned tile shapes, hipGraph steps."""
+    probs = []
+    for prec in (tecno.FP64, tecno.FP32):
+        for n in ENS_1D_N:
+            probs.append(dict(kind="burgers1d", n=n, prec=prec, t=ENS_T1D,
+                              cfg=tecno.make_cfg(system="scalar", flux="bur_tecno4", recon="sp_weno",
+                                                 n=(n, 1, 1), dims=1, lo=(-np.pi, 0, 0),
+                                                 hi=(np.pi, 1, 1), bc="periodic", precision=prec)))
+        for kind, flux in (("burgers2d", "bur_tecno4"), ("linadv2d", "lin_tecno4")):
+            for n in ENS_2D_N:
+                probs.append(dict(kind=kind, n=n, prec=prec, t=ENS_T2D,
+                                  cfg=tecno.make_cfg(system="scalar", flux=flux, recon="sp_weno",
+                                                     n=(n, n, 1), dims=2, lo=(-1, -1, 0),
+                                                     hi=(1, 1, 1), vel=(1, 1, 0), bc="periodic",
+                                                     precision=prec)))
+    ens = tecno.Ensemble([p["cfg"] for p in probs])
+
+    def init():
+        for m, p in zip(ens.members, probs):
+            if p["kind"] == "burgers1d":
+                h = 2 * np.pi / p["n"]
+                m.set_state(np.sin(-np.pi + (np.arange(p["n"]) + 0.5) * h))
+            else:
+                f, X, Y = ens_ic2d(p["n"])
+                m.set_state(f(X, Y).ravel())
+        for m in ens.members:
+            m.sync()
+    init()
+    ens.autotune(steps=3)
+    # untimed warm-up pass (graph capture, code load), then the timed pass
+    ens.advance([p["t"] for p in probs], use_graph=use_graph)
+    init()
+    t0 = time.perf_counter()
+    cu = ens.advance([p["t"] for p in probs], use_graph=use_graph)
+    el = time.perf_counter() - t0
+    # errors: linear advection and 1D Burgers against exact solutions (cell
+    # centre values, as compute_errors.cc); 2D Burgers against the 1024^2
+    # solution averaged onto the coarser meshes that divide it
+    res = []
+    fine = {}
+    for m, p in zip(ens.members, probs):
+        if p["kind"] == "burgers2d" and p["n"] == 1024:
+            fine[p["prec"]] = m.get_state().reshape(1024, 1024)
+    for m, p in zip(ens.members, probs):
+        t, it = m.time()
+        n = p["n"]
+        if p["kind"] == "burgers1d":
+            h = 2 * np.pi / n
+            x = -np.pi + (np.arange(n) + 0.5) * h
+            err = float(m.l1_error(burgers_sine_exact(x, t))[0])
+        elif p["kind"] == "linadv2d":
+            f, X, Y = ens_ic2d(n)
+            Xs = (X - t + 1) % 2 - 1
+            Ys = (Y - t + 1) % 2 - 1
+            err = float(m.l1_error(f(Xs, Ys).ravel())[0])
+        elif 1024 % n == 0 and n < 1024:
+            r = 1024 // n
+            ref = fine[p["prec"]].reshape(n, r, n, r).mean(axis=(1, 3))
+            err = float(m.l1_error(ref.ravel())[0])
+        else:
+            err = None
+        res.append(dict(kind=p["kind"], n=n, precision="fp32" if p["prec"] == tecno.FP32 else "fp64",
+                        steps=it, time=t, l1=err))
+    orders = {}
+    for kind in ("burgers1d", "linadv2d", "burgers2d"):
+        for prec in ("fp64", "fp32"):
+            rows = [r for r in res if r["kind"] == kind and r["precision"] == prec and r["l1"]]
+            orders[f"{kind}_{prec}"] = [
+                round(float(np.log(a["l1"] / b["l1"]) / np.log(b["n"] / a["n"])), 3)
+                for a, b in zip(rows[:-1], rows[1:])]
+    groups = ens.groups()
+    ens.close()
+    return dict(cell_updates=cu, seconds=el, cell_updates_per_s=cu / el, problems=res,
+                orders=orders, groups=[dict(g, tuned_ms={f"{k[0]}x{k[1]}": v for k, v in
+                                                           g["tuned_ms"].items()}) for g in groups])
+
+
 def load_profile(kind, variant):
     p = os.path.join(ROOT, "profiles", f"{kind}_{variant}.json")
     if os.path.exists(p):
@@ -237,6 +360,9 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="headline | fp64 | fp64dsp (profiling runs)")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="run the 48-problem scalar ensemble (batched, hipGraph) and exit")
+    ap.add_argument("--no-graph", action="store_true", help="ensemble without the hipGraph")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one plain 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -255,6 +381,18 @@ def main():
               f"(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     import tecno
+
+    if args.ensemble:
+        r = ensemble_bench(tecno, use_graph=not args.no_graph)
+        print(json.dumps({"metric": "cell-updates/s, 48-problem scalar ensemble (config 4)",
+                          "value": r["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
+                          "higher_is_better": True, "data": "synthetic (seeded initial states)",
+                          "dtype": "fp64 and fp32 members", "hipgraph": not args.no_graph,
+                          "config": {"workload": "6 x 1D Burgers sin x (t=1.5), 9 x 2D Burgers, 9 x 2D "
+                                                 "linear advection (t=0.5), x {fp64, fp32}; "
+                                                 "bur_tecno4 / lin_tecno4 + sp_weno, SSPRK3 CFL 0.5"},
+                          **r}), flush=True)
+        return
 
     n = args.n
     head_prec, head_rec, head_mlp = tecno.FP32, "sp_weno_dl", tecno.MLP_BF16

@@ -62,11 +62,16 @@ class Cfg(C.Structure):
     ]
 
 
+MU_CONSTANT, MU_SUTHERLAND, MU_POWER = 0, 1, 2
+
+
 def make_cfg(*, system="euler", flux="kepes_tecno4_roe", recon="sp_weno", n=(32, 32, 2),
              lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dims=2, bc="periodic", cfl=0.5,
              vel=(1.0, 0.0, 0.0), model=None, precision=FP64, mlp_dtype=MLP_FP64,
              gamma=1.4, gas_const=1.0, ec1_fix=0, fix_zmom=0, sign_fallback=0,
-             time_scheme=TS_SSPRK3):
+             time_scheme=TS_SSPRK3, viscosity=("constant", 0.0), prandtl=0.72):
+    """viscosity: ("constant", mu_ref) | ("sutherland", mu_ref, T_ref, T_0) |
+    ("power", mu_ref, T_ref, omega), as param.in's material/viscosity block"""
     c = Cfg()
     if system == "scalar":
         c.system = SYS_SCALAR
@@ -94,7 +99,14 @@ def make_cfg(*, system="euler", flux="kepes_tecno4_roe", recon="sp_weno", n=(32,
     c.cfl = cfl
     c.gamma = gamma
     c.gas_const = gas_const
-    c.prandtl = 0.72
+    c.prandtl = prandtl
+    vm = viscosity[0]
+    c.mu_model = {"constant": MU_CONSTANT, "sutherland": MU_SUTHERLAND, "power": MU_POWER}[vm]
+    c.mu_ref = float(viscosity[1])
+    if vm == "sutherland":
+        c.T_ref, c.T_0 = float(viscosity[2]), float(viscosity[3])
+    elif vm == "power":
+        c.T_ref, c.omega = float(viscosity[2]), float(viscosity[3])
     c.ec1_fix = int(ec1_fix)
     c.fix_zmom = int(fix_zmom)
     c.sign_fallback = int(sign_fallback)

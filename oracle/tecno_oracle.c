@@ -902,6 +902,20 @@ void tor_destroy(tor_solver* s) {
 int64_t tor_num_cells(const tor_solver* s) { return s->ncell; }
 int tor_num_vars(const tor_solver* s) { return s->nvar; }
 
+/* Parameter::Viscosity parameter.cc:646-663 */
+static double viscosity(const tecno_cfg* c, double T) {
+  switch (c->mu_model) {
+    case TECNO_MU_SUTHERLAND:
+      return c->mu_ref * pow(T / c->T_ref, 1.5) * (c->T_ref + c->T_0) / (T + c->T_0);
+    case TECNO_MU_POWER: return c->mu_ref * pow(T / c->T_ref, c->omega);
+    default: return c->mu_ref;
+  }
+}
+static int is_ns(const tecno_cfg* c) {
+  return c->system == TECNO_SYS_EULER &&
+         (c->model == TECNO_MODEL_NS_ES || c->model == TECNO_MODEL_NS_KEP);
+}
+
 /* ghost fill.  scalar: enflo_scalar fv.cc:110-281; euler: enflo fv.cc:121-361
  * (primitive variables; edges/corners are only needed by the viscous terms and
  * are filled periodically as the reference does). */
@@ -968,6 +982,19 @@ static void euler_bc(tor_solver* s) {
             s->P[pidx(s, cg[0], cg[1], cg[2])] = p;
           }
     }
+  }
+  if (is_ns(&s->c)) {
+    /* edges and corners, periodic images (fv.cc:317-359; the viscous node
+     * operators read one layer of them) */
+    for (int k = -1; k <= s->n[2]; ++k)
+      for (int j = -1; j <= s->n[1]; ++j)
+        for (int i = -1; i <= s->n[0]; ++i) {
+          int out = (i < 0 || i >= s->n[0]) + (j < 0 || j >= s->n[1]) + (k < 0 || k >= s->n[2]);
+          if (out < 2) continue;
+          int si = (i + s->n[0]) % s->n[0], sj = (j + s->n[1]) % s->n[1],
+              sk = (k + s->n[2]) % s->n[2];
+          s->P[pidx(s, i, j, k)] = s->P[pidx(s, si, sj, sk)];
+        }
   }
 }
 
@@ -1040,6 +1067,9 @@ static double compute_dt(tor_solver* s) {
         } else {
           const prim_t* P = &s->P[pidx(s, i, j, k)];
           speed = sqrt(vsq(P->u)) + sqrt(s->c.gamma * P->p / P->rho);
+          if (is_ns(&s->c)) /* fv.cc:375-377 */
+            speed += viscosity(&s->c, P->p / (s->c.gas_const * P->rho)) / P->rho /
+                     dmin(s->g.h[0], dmin(s->g.h[1], s->g.h[2]));
         }
         dt = dmin(dt, s->hmin / speed);
       }
@@ -1134,6 +1164,175 @@ static void euler_residual(tor_solver* s) {
   }
 }
 
+/* ---- Navier-Stokes viscous terms (periodic grids) --------------------- */
+typedef struct { double m[5]; } flux5; /* mass, mom x/y/z, energy */
+
+/* nodal_vlux_ES vflux_ES.cc:15-151, including its ordering: the K21 and K31,
+ * K32 contributions to the y and z fluxes are overwritten by the K22 / K33
+ * assignments, so y = K22 V_y + K23 V_z and z = K33 V_z. */
+static void node_flux_es(const tor_solver* s, int i, int j, int k, flux5* X, flux5* Y, flux5* Z) {
+  const tecno_cfg* c = &s->c;
+  const double dx = s->g.h[0], dy = s->g.h[1], dz = s->g.h[2];
+  /* cells: index (a,b,c) in {0,1}^3 -> V[pidx(i+a, j+b, k+c)] */
+#define VV(a, b, cc) (&s->V[pidx(s, i + (a), j + (b), k + (cc))])
+  const ent_t* E[2][2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int q = 0; q < 2; ++q) E[a][b][q] = VV(a, b, q);
+#undef VV
+  double avg[5], Tx[5], Ty[5], Tz[5];
+  const double f1 = 1.0 / 8.0, fx = 1.0 / 4.0 / dx, fy = 1.0 / 4.0 / dy, fz = 1.0 / 4.0 / dz;
+  for (int comp = 0; comp < 5; ++comp) {
+#define C(e) ((comp == 0) ? (e)->f : (comp == 4) ? (e)->l : (e)->v[comp - 1])
+    /* ent_avg: (111)(011)(101)(110) then (001)(010)(100)(000) */
+    avg[comp] = C(E[1][1][1]) * f1 + C(E[0][1][1]) * f1 + C(E[1][0][1]) * f1 + C(E[1][1][0]) * f1;
+    avg[comp] += C(E[0][0][1]) * f1 + C(E[0][1][0]) * f1 + C(E[1][0][0]) * f1 + C(E[0][0][0]) * f1;
+    Tx[comp] = C(E[1][1][1]) * fx + C(E[1][1][0]) * fx + C(E[1][0][1]) * fx + C(E[1][0][0]) * fx;
+    Tx[comp] += C(E[0][1][1]) * -fx + C(E[0][1][0]) * -fx + C(E[0][0][1]) * -fx + C(E[0][0][0]) * -fx;
+    Ty[comp] = C(E[1][1][1]) * fy + C(E[1][1][0]) * fy + C(E[0][1][1]) * fy + C(E[0][1][0]) * fy;
+    Ty[comp] += C(E[1][0][1]) * -fy + C(E[1][0][0]) * -fy + C(E[0][0][1]) * -fy + C(E[0][0][0]) * -fy;
+    Tz[comp] = C(E[1][1][1]) * fz + C(E[1][0][1]) * fz + C(E[0][1][1]) * fz + C(E[0][0][1]) * fz;
+    Tz[comp] += C(E[1][1][0]) * -fz + C(E[1][0][0]) * -fz + C(E[0][1][0]) * -fz + C(E[0][0][0]) * -fz;
+#undef C
+  }
+  const double T_avg = -1 / avg[4] / c->gas_const;
+  const double mu = viscosity(c, T_avg);
+  const double Cp = c->gamma * c->gas_const / (c->gamma - 1.0); /* parameter.cc:286 */
+  const double kappa = mu * Cp / c->prandtl;
+  const double c1 = 4.0 * mu / 3.0, c2 = -2.0 * mu / 3.0, c3 = kappa / c->gas_const,
+               e4 = 1.0 / avg[4], e1 = avg[1] * e4, e2 = avg[2] * e4, e3 = avg[3] * e4;
+  /* X: K11 V_x + K12 V_y + K13 V_z */
+  X->m[0] = 0.0;
+  X->m[1] = c1 * e4 * (-Tx[1] + e1 * Tx[4]);
+  X->m[2] = mu * e4 * (-Tx[2] + e2 * Tx[4]);
+  X->m[3] = mu * e4 * (-Tx[3] + e3 * Tx[4]);
+  X->m[4] = e4 * (c1 * e1 * Tx[1] + mu * e2 * Tx[2] + mu * e3 * Tx[3] +
+                  (-c1 * e1 * e1 - mu * (e2 * e2 + e3 * e3) + c3 * e4) * Tx[4]);
+  X->m[1] += c2 * e4 * (-Ty[2] + e2 * Ty[4]);
+  X->m[2] += mu * e4 * (-Ty[1] + e1 * Ty[4]);
+  X->m[3] += 0;
+  X->m[4] += e4 * (mu * e2 * Ty[1] + c2 * e1 * Ty[2] - (mu + c2) * e1 * e2 * Ty[4]);
+  X->m[1] += c2 * e4 * (-Tz[3] + e3 * Tz[4]);
+  X->m[2] += 0;
+  X->m[3] += mu * e4 * (-Tz[1] + e1 * Tz[4]);
+  X->m[4] += e4 * (mu * e3 * Tz[1] + c2 * e1 * Tz[3] - (mu + c2) * e1 * e3 * Tz[4]);
+  /* Y: K22 V_y (assigned), K23 V_z */
+  Y->m[0] = 0.0;
+  Y->m[1] = mu * e4 * (-Ty[1] + e1 * Ty[4]);
+  Y->m[2] = c1 * e4 * (-Ty[2] + e2 * Ty[4]);
+  Y->m[3] = mu * e4 * (-Ty[3] + e3 * Ty[4]);
+  Y->m[4] = e4 * (mu * e1 * Ty[1] + c1 * e2 * Ty[2] + mu * e3 * Ty[3] +
+                  (-c1 * e2 * e2 - mu * (e1 * e1 + e3 * e3) + c3 * e4) * Ty[4]);
+  Y->m[1] += 0;
+  Y->m[2] += c2 * e4 * (-Tz[3] + e3 * Tz[4]);
+  Y->m[3] += mu * e4 * (-Tz[2] + e2 * Tz[4]);
+  Y->m[4] += e4 * (mu * e3 * Tz[2] + c2 * e2 * Tz[3] - (mu + c2) * e2 * e3 * Tz[4]);
+  /* Z: K33 V_z (assigned) */
+  Z->m[0] = 0.0;
+  Z->m[1] = mu * e4 * (-Tz[1] + e1 * Tz[4]);
+  Z->m[2] = mu * e4 * (-Tz[2] + e2 * Tz[4]);
+  Z->m[3] = c1 * e4 * (-Tz[3] + e3 * Tz[4]);
+  Z->m[4] = e4 * (mu * e1 * Tz[1] + mu * e2 * Tz[2] + c1 * e3 * Tz[3] +
+                  (-c1 * e3 * e3 - mu * (e1 * e1 + e2 * e2) + c3 * e4) * Tz[4]);
+  const double ax = -(dy * dz), ay = -(dx * dz), az = -(dx * dy);
+  for (int v = 0; v < 5; ++v) {
+    X->m[v] *= ax;
+    Y->m[v] *= ay;
+    Z->m[v] *= az;
+  }
+}
+
+/* nodal_vlux_KE vflux_KE.cc:15-96 with nodal_avg_and_vel_derivatives
+ * nodal_avg_and_der.cc:16-82 */
+static void node_flux_ke(const tor_solver* s, int i, int j, int k, flux5* X, flux5* Y, flux5* Z) {
+  const tecno_cfg* c = &s->c;
+  const double dx = s->g.h[0], dy = s->g.h[1], dz = s->g.h[2], R = c->gas_const;
+  const prim_t* P[2][2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int q = 0; q < 2; ++q) P[a][b][q] = &s->P[pidx(s, i + a, j + b, k + q)];
+  const double f1 = 1.0 / 8.0, fx = 1.0 / (4.0 * dx), fy = 1.0 / (4.0 * dy), fz = 1.0 / (4.0 * dz);
+  double avg[5];
+  for (int comp = 0; comp < 5; ++comp) {
+#define C(q_) ((comp == 0) ? (q_)->rho : (comp == 4) ? (q_)->p : (q_)->u[comp - 1])
+    avg[comp] = C(P[1][1][1]) * f1 + C(P[0][1][1]) * f1 + C(P[1][0][1]) * f1 + C(P[1][1][0]) * f1;
+    avg[comp] += C(P[0][0][1]) * f1 + C(P[0][1][0]) * f1 + C(P[1][0][0]) * f1 + C(P[0][0][0]) * f1;
+#undef C
+  }
+  double Tx[3], Ty[3], Tz[3];
+  for (int d = 0; d < 3; ++d) {
+    Tx[d] = P[1][1][1]->u[d] * fx + P[1][1][0]->u[d] * fx + P[1][0][1]->u[d] * fx + P[1][0][0]->u[d] * fx;
+    Tx[d] += P[0][1][1]->u[d] * -fx + P[0][1][0]->u[d] * -fx + P[0][0][1]->u[d] * -fx + P[0][0][0]->u[d] * -fx;
+    Ty[d] = P[1][1][1]->u[d] * fy + P[1][1][0]->u[d] * fy + P[0][1][1]->u[d] * fy + P[0][1][0]->u[d] * fy;
+    Ty[d] += P[1][0][1]->u[d] * -fy + P[1][0][0]->u[d] * -fy + P[0][0][1]->u[d] * -fy + P[0][0][0]->u[d] * -fy;
+    Tz[d] = P[1][1][1]->u[d] * fz + P[1][0][1]->u[d] * fz + P[0][1][1]->u[d] * fz + P[0][0][1]->u[d] * fz;
+    Tz[d] += P[1][1][0]->u[d] * -fz + P[1][0][0]->u[d] * -fz + P[0][1][0]->u[d] * -fz + P[0][0][0]->u[d] * -fz;
+  }
+#define TT(c_) ((c_)->p / (R * (c_)->rho))
+  const double T_avg = avg[4] / (R * avg[0]);
+  const double mu = viscosity(c, T_avg);
+  const double Cp = c->gamma * R / (c->gamma - 1.0);
+  const double kappa = mu * Cp / c->prandtl;
+  const double Txt = fx * (TT(P[1][1][1]) + TT(P[1][1][0]) + TT(P[1][0][1]) + TT(P[1][0][0]) -
+                           TT(P[0][1][1]) - TT(P[0][1][0]) - TT(P[0][0][1]) - TT(P[0][0][0]));
+  const double Tyt = fy * (TT(P[1][1][1]) + TT(P[1][1][0]) + TT(P[0][1][1]) + TT(P[0][1][0]) -
+                           TT(P[1][0][1]) - TT(P[1][0][0]) - TT(P[0][0][1]) - TT(P[0][0][0]));
+  const double Tzt = fz * (TT(P[1][1][1]) + TT(P[1][0][1]) + TT(P[0][1][1]) + TT(P[0][0][1]) -
+                           TT(P[1][1][0]) - TT(P[1][0][0]) - TT(P[0][1][0]) - TT(P[0][0][0]));
+#undef TT
+  const double gv = Tx[0] + Ty[1] + Tz[2];
+  const double txx = mu * (2 * Tx[0] - (2.0 / 3.0) * gv), tyy = mu * (2 * Ty[1] - (2.0 / 3.0) * gv),
+               tzz = mu * (2 * Tz[2] - (2.0 / 3.0) * gv), txy = mu * (Ty[0] + Tx[1]),
+               txz = mu * (Tx[2] + Tz[0]), tyz = mu * (Ty[2] + Tz[1]);
+  const double* u = &avg[1];
+  X->m[0] = 0.0; X->m[1] = txx; X->m[2] = txy; X->m[3] = txz;
+  X->m[4] = u[0] * txx + u[1] * txy + u[2] * txz + kappa * Txt;
+  Y->m[0] = 0.0; Y->m[1] = txy; Y->m[2] = tyy; Y->m[3] = tyz;
+  Y->m[4] = u[0] * txy + u[1] * tyy + u[2] * tyz + kappa * Tyt;
+  Z->m[0] = 0.0; Z->m[1] = txz; Z->m[2] = tyz; Z->m[3] = tzz;
+  Z->m[4] = u[0] * txz + u[1] * tyz + u[2] * tzz + kappa * Tzt;
+  const double ax = -(dy * dz), ay = -(dx * dz), az = -(dx * dy);
+  for (int v = 0; v < 5; ++v) {
+    X->m[v] *= ax;
+    Y->m[v] *= ay;
+    Z->m[v] *= az;
+  }
+}
+
+/* compute_visc_residual fv.cc:786-1157 on a periodic grid: every node
+ * (i+1/2, j+1/2, k+1/2), i = -1 .. n-1, scatters its three fluxes to the
+ * (up to 8) interior cells around it with weights (-1)^l/4 per direction
+ * (Flux::sadd of 3 fluxes).  The reference visits interior, face and corner
+ * nodes once, but every edge node (two coordinates on the boundary) twice:
+ * once in each of the two boundary groups that share the edge (e.g. node
+ * (Nx-1, j, -1) in the "X-Right" and the "Z-Left" loops), so edge nodes add
+ * their fluxes twice. */
+static void visc_residual(tor_solver* s) {
+  const int es = s->c.model == TECNO_MODEL_NS_ES;
+  for (int k = -1; k < s->n[2]; ++k)
+    for (int j = -1; j < s->n[1]; ++j)
+      for (int i = -1; i < s->n[0]; ++i) {
+        flux5 X, Y, Z;
+        if (es)
+          node_flux_es(s, i, j, k, &X, &Y, &Z);
+        else
+          node_flux_ke(s, i, j, k, &X, &Y, &Z);
+        const int nb = (i == -1 || i == s->n[0] - 1) + (j == -1 || j == s->n[1] - 1) +
+                       (k == -1 || k == s->n[2] - 1);
+        for (int rep = 0; rep < (nb == 2 ? 2 : 1); ++rep)
+        for (int l = 0; l < 2; ++l)
+          for (int m = 0; m < 2; ++m)
+            for (int q = 0; q < 2; ++q) {
+              int ci = i + l, cj = j + m, ck = k + q;
+              if (ci < 0 || cj < 0 || ck < 0 || ci >= s->n[0] || cj >= s->n[1] || ck >= s->n[2])
+                continue;
+              double a = (l ? -1.0 : 1.0) / 4.0, b = (m ? -1.0 : 1.0) / 4.0, cc = (q ? -1.0 : 1.0) / 4.0;
+              double* r = &s->res[5 * iidx(s, ci, cj, ck)];
+              for (int v = 0; v < 5; ++v) r[v] += X.m[v] * a + Y.m[v] * b + Z.m[v] * cc;
+            }
+      }
+}
+
 static const double ARK[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0}; /* ext_constants.h:7 */
 static const double BRK[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0}; /* ext_constants.h:8 */
 
@@ -1163,6 +1362,7 @@ int tor_step(tor_solver* s, double t_stamp, double* dt_out) {
     for (int rk = 0; rk < nrk; ++rk) {
       euler_prim_to_ent_all(s);
       euler_residual(s);
+      if (is_ns(&s->c)) visc_residual(s); /* compute_residual fv.cc:591-603 */
       /* update_solution: enflo fv.cc:1163-1175 (ConVar::sadd, convar.h:118) */
       double c2 = -factor * BRK[rk];
       for (int64_t q = 0; q < N; ++q)

@@ -76,6 +76,14 @@ def euler_cfg(key, **kw):
     if key == "eu3d_tg":
         return make_cfg(lo=(0.0, 0.0, 0.0), hi=(2 * math.pi,) * 3, n=(8, 6, 6), dims=3, recon="sp_weno",
                         flux="kepes_tecno4_roe", bc="periodic", **kw), 1.0, 5
+    if key.startswith("ns3d_"):
+        spec = {"ns3d_es_tg": ("ns_es", ("constant", 1.0 / 1600), "sp_weno", (10, 8, 6)),
+                "ns3d_kep_tg": ("ns_kep", ("constant", 1.0 / 1600), "sp_weno", (8, 10, 6)),
+                "ns3d_es_suth": ("ns_es", ("sutherland", 0.05, 71.0, 40.0), "sp_weno", (6, 10, 8)),
+                "ns3d_es_dsp": ("ns_es", ("power", 0.02, 70.0, 0.7), "sp_weno_dl", (8, 8, 8))}[key]
+        return make_cfg(lo=(0.0, 0.0, 0.0), hi=(2 * math.pi,) * 3, n=spec[3], dims=3, recon=spec[2],
+                        flux="kepes_tecno4_roe", bc="periodic", model=spec[0], viscosity=spec[1],
+                        **kw), 1.0, 5
     if key == "eu2d_wall":
         return make_cfg(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(16, 12, 2), dims=2, recon="sp_weno",
                         flux="kepes_tecno4_roe",

@@ -31,7 +31,8 @@ BC_NAMES = ["xleft", "xright", "yleft", "yright", "zleft", "zright"]
 def param_in(*, system, lo, hi, n, recon, flux, final_time, ic, bc="periodic",
              cfl=0.5, t_stamps=5, pdim=1, vel=(1, 0, 0), model=None,
              constants=(), net=NET_FIXTURE, ec1_fix="no", out=".",
-             max_iter=50000, exact=None, ooc="no"):
+             max_iter=50000, exact=None, ooc="no", euler_model="euler",
+             viscosity=("constant", 0.0), prandtl=0.72):
     """Text of a param.in file (sections in the order Parameter::read needs)."""
     L = ["grid", "{"]
     for d, ax in enumerate("xyz"):
@@ -48,9 +49,15 @@ def param_in(*, system, lo, hi, n, recon, flux, final_time, ic, bc="periodic",
             L += ["   model       linadv", f"   vel         {{ {vel[0]!r} {vel[1]!r} {vel[2]!r} }}"]
         L += [f"   flux        {flux}"]
     else:
-        L += ["   gamma       1.4", "   gas_const   1.0", "   viscosity", "   {",
-              "      model    constant", "      mu_ref   0.0", "   }", "   prandtl     0.72",
-              "   model       euler", f"   flux        {flux}", f"   EC1_fix     {ec1_fix}"]
+        vm = viscosity[0]
+        vl = [f"      model    {vm}", f"      mu_ref   {viscosity[1]!r}"]
+        if vm == "sutherland":
+            vl += [f"      T_ref    {viscosity[2]!r}", f"      T_0      {viscosity[3]!r}"]
+        elif vm == "power":
+            vl += [f"      T_ref    {viscosity[2]!r}", f"      omega    {viscosity[3]!r}"]
+        L += ["   gamma       1.4", "   gas_const   1.0", "   viscosity", "   {"] + vl + [
+              "   }", f"   prandtl     {prandtl!r}",
+              f"   model       {euler_model}", f"   flux        {flux}", f"   EC1_fix     {ec1_fix}"]
     L += ["}", "", "constants", "{"]
     for k, v in constants:
         L += [f"   {k}   {v!r}"]
@@ -332,6 +339,22 @@ def euler_cases(tmp):
                     bc="periodic", out=tmp)
     init, t, dt, st = harness_run(REFH_SYS, text, 3, tmp)
     fx["eu3d_tg"] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
+    # Navier-Stokes Taylor-Green vortex (Mach 0.1 pressure field, Re = 1600
+    # at unit scales) on uneven periodic boxes: ES and KE viscous fluxes,
+    # constant and Sutherland viscosity, sp_weno and sp_weno_dl.  Ny >= Nz on
+    # every box: the reference's node-weight diagnostic indexes with Nz in
+    # place of Ny (fv.cc:1147,1154) and leaves its array when Nz > Ny
+    tgv = ["1.0", "sin(x)*cos(y)*cos(z)", "-cos(x)*sin(y)*cos(z)", "0.0",
+           "71.42857142857143 + 0.0625*(cos(2*x)+cos(2*y))*(cos(2*z)+2)"]
+    for tag, em, visc, rec, n in [("ns3d_es_tg", "ns_es", ("constant", 1.0 / 1600), "sp_weno", (10, 8, 6)),
+                                  ("ns3d_kep_tg", "ns_kep", ("constant", 1.0 / 1600), "sp_weno", (8, 10, 6)),
+                                  ("ns3d_es_suth", "ns_es", ("sutherland", 0.05, 71.0, 40.0), "sp_weno", (6, 10, 8)),
+                                  ("ns3d_es_dsp", "ns_es", ("power", 0.02, 70.0, 0.7), "sp_weno_dl", (8, 8, 8))]:
+        text = param_in(system="euler", lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, n=n, recon=rec,
+                        flux="kepes_tecno4_roe", final_time=1.0, ic=tgv, pdim=3, bc="periodic", out=tmp,
+                        euler_model=em, viscosity=visc)
+        init, t, dt, st = harness_run(REFH_SYS, text, 3, tmp)
+        fx[tag] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
     text = param_in(system="euler", lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(16, 12, 2),
                     recon="sp_weno", flux="kepes_tecno4_roe", final_time=0.25, ic=ic, constants=consts,
                     bc=["wall", "wall", "neumann", "wall", "periodic", "periodic"], pdim=2, out=tmp)

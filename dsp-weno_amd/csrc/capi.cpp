@@ -145,6 +145,13 @@ static void fill_phys(const tecno_ctx* c, Phys<T>& P) {
   P.sgm1_d = std::sqrt(g.gamma - 1.0);
   P.inv_gamma_d = 1.0 / g.gamma;
   P.vel_norm = std::sqrt(g.vel[0] * g.vel[0] + g.vel[1] * g.vel[1] + g.vel[2] * g.vel[2]);
+  P.mu_model = g.mu_model;
+  P.mu_ref = g.mu_ref;
+  P.T_ref = g.T_ref;
+  P.T_0 = g.T_0;
+  P.omega = g.omega;
+  P.prandtl = g.prandtl;
+  P.Cp = g.gamma * g.gas_const / (g.gamma - 1.0);  // parameter.cc:286
   P.arch5 = 0;
   P.net.nl = c->net.nl;
   for (int i = 0; i <= TECNO_NET_MAXL; ++i) P.net.w[i] = c->net.w[i];
@@ -276,16 +283,34 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   *out = nullptr;
   const tecno_cfg& g = *cfg;
   if (g.dims < 1 || g.dims > 3) return fail(TECNO_EINVAL, "dims must be 1, 2 or 3");
-  if (g.system == TECNO_SYS_EULER && g.dims != 2)
-    return fail(TECNO_EINVAL, "Euler: only 2D grids are supported by this build");
+  if (g.system == TECNO_SYS_EULER && g.dims == 1)
+    return fail(TECNO_EINVAL, "Euler: 2D or 3D grids");
+  if (g.system == TECNO_SYS_EULER && g.dims == 3) {
+    if (g.precision != TECNO_FP64)
+      return fail(TECNO_EINVAL, "3D Euler / Navier-Stokes runs in FP64 in this build");
+    if (g.recon == TECNO_REC_ENO && g.eno_level == 3)
+      return fail(TECNO_EINVAL, "3D: reconstructions with 2 ghost cells (eno 3 is 2D only)");
+  }
+  const bool ns = g.model == TECNO_MODEL_NS_ES || g.model == TECNO_MODEL_NS_KEP;
+  if (g.system == TECNO_SYS_EULER && ns) {
+    if (g.dims != 3) return fail(TECNO_EINVAL, "Navier-Stokes runs on 3D grids");
+    for (int i = 0; i < 6; ++i)
+      if (g.bc[i] != TECNO_BC_PERIODIC)
+        return fail(TECNO_EINVAL, "Navier-Stokes: periodic boundaries only (parameter.cc:564-568)");
+    for (int d = 0; d < 3; ++d)
+      if (g.n[d] < 2) return fail(TECNO_EINVAL, "Navier-Stokes: at least 2 cells per direction");
+    if (!(g.prandtl > 0.0)) return fail(TECNO_EINVAL, "prandtl must be positive");
+    if (g.mu_model < TECNO_MU_CONSTANT || g.mu_model > TECNO_MU_POWER)
+      return fail(TECNO_EINVAL, "unknown viscosity model");
+  }
   if (g.system == TECNO_SYS_SCALAR && g.dims == 3)
     return fail(TECNO_EINVAL, "scalar: 3D grids are not supported by this build");
   if (g.system == TECNO_SYS_SCALAR && (g.flux < 0 || g.flux > 7))
     return fail(TECNO_EINVAL, "unknown scalar flux");
   if (g.system == TECNO_SYS_EULER && !((g.flux >= 100 && g.flux <= 106) || (g.flux >= 120 && g.flux <= 124)))
     return fail(TECNO_EINVAL, "unknown Euler flux");
-  if (g.system == TECNO_SYS_EULER && g.model != TECNO_MODEL_EULER)
-    return fail(TECNO_EINVAL, "Navier-Stokes models are not supported by this build");
+  if (g.system == TECNO_SYS_EULER && g.model != TECNO_MODEL_EULER && !ns)
+    return fail(TECNO_EINVAL, "unknown Euler model");
   if (g.recon == TECNO_REC_ENO && g.eno_level != 2 && g.eno_level != 3)
     return fail(TECNO_EINVAL, "eno k: k must be 2 or 3");
   if (g.time_scheme != TECNO_TS_SSPRK3 && g.time_scheme != TECNO_TS_RK1)

@@ -373,6 +373,9 @@ struct Phys {
   double vol_d;       // dx*dy*dz (param.in spacings)
   double hmin_d, cfl_d;
   double vel_norm;    // |vel| (dt of linear advection)
+  // Navier-Stokes (model 11 ns_kep, 12 ns_es): Parameter::Viscosity and Cp
+  int mu_model;       // 0 constant, 1 sutherland, 2 power
+  double mu_ref, T_ref, T_0, omega, prandtl, Cp;
   NetView<T> net;
 };
 

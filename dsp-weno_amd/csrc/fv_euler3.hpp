@@ -1,0 +1,623 @@
+// fv_euler3.hpp — 3D Euler / Navier-Stokes stage in FP64 (reference
+// arithmetic): the TeCNO and EC fluxes in the global frame with the unit axis
+// normal (kepes_tecno4_roe.cc:10-176 and variants, kepec.cc, roe_ec.cc,
+// kep.cc) and the node-centred viscous fluxes of compute_visc_residual
+// (fv.cc:786-1157, vflux_ES.cc:15-151, vflux_KE.cc:15-96).
+//
+// A workgroup owns an 8x8x2 tile and loads the whole (TX+2G)x(TY+2G)x(TZ+2G)
+// box of cell records into LDS (edges and corners included: the viscous node
+// operators read the 8 cells around each node).  Face fluxes are evaluated
+// one per lane per round; the (TX+1)(TY+1)(TZ+1) viscous node fluxes of the
+// tile, one per lane; both go to LDS over the dead records; each cell then
+// gathers its 6 faces and 8 corner nodes, applies the SSPRK3 update and
+// reduces the CFL speed.
+#pragma once
+#include "fv_kernels.hpp"
+
+namespace tecno {
+
+// record layout, double [E3_NREC][NB]: primitive, beta as kepec_flux computes
+// it, entropy variables (fv.cc:421-431)
+enum { E3_RHO = 0, E3_U, E3_V, E3_W, E3_P, E3_BK, E3_VF, E3_V1, E3_V2, E3_V3, E3_VL, E3_NREC };
+
+template <typename T>
+struct Pr3 {
+  T rho, u[3], p, bk;
+};
+
+// Parameter::Viscosity parameter.cc:646-663
+template <typename T>
+__device__ __forceinline__ double visc_mu(const Phys<T>& P, double Tm) {
+  if (P.mu_model == 1) return P.mu_ref * pow(Tm / P.T_ref, 1.5) * (P.T_ref + P.T_0) / (Tm + P.T_0);
+  if (P.mu_model == 2) return P.mu_ref * pow(Tm / P.T_ref, P.omega);
+  return P.mu_ref;
+}
+
+template <typename T>
+__device__ __forceinline__ T vdot3(const T a[3], const T b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// kepec_flux kepec.cc:10-48
+template <typename T>
+__device__ __forceinline__ void kepec3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,
+                                       const T n[3], T area, T F[5]) {
+  const T rho = logavg(l.rho, r.rho);
+  const T vel2 = T(0.5) * (vdot3(l.u, l.u) + vdot3(r.u, r.u));
+  const T beta = logavg(l.bk, r.bk);
+  const T p = T(0.5) * (l.rho + r.rho) / (l.bk + r.bk);
+  T vel[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) vel[d] = l.u[d] * T(0.5) + r.u[d] * T(0.5);
+  const T vn = vdot3(vel, n);
+  const T mass = rho * vn;
+  T mom[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * p + vel[d] * mass;
+  const T en = T(0.5) * (T(1) / ((P.gamma - T(1)) * beta) - vel2) * mass + vdot3(mom, vel);
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+// roe_ec_flux roe_ec.cc:10-69
+template <typename T>
+__device__ __forceinline__ void roeec3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,
+                                       const T n[3], T area, T F[5]) {
+  const T bl = m_sqrt(l.rho / l.p), br = m_sqrt(r.rho / r.p);
+  const T z1b = T(0.5) * (bl + br);
+  T zvb[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) zvb[d] = l.u[d] * (T(0.5) * bl) + r.u[d] * (T(0.5) * br);
+  const T z5b = T(0.5) * (bl * l.p + br * r.p);
+  const T zvn = vdot3(zvb, n);
+  const T z1l = logavg(bl, br);
+  const T z5l = logavg(bl * l.p, br * r.p);
+  const T mass = z5l * zvn;
+  T mom[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * (z5b / z1b) + zvb[d] * (mass / z1b);
+  const T en = T(0.5) * ((P.gamma + T(1)) / (P.gamma - T(1)) * mass / z1l + vdot3(mom, zvb)) / z1b;
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+// kep_flux kep.cc:10-44 with Enthalpy fv.cc:565-569
+template <typename T>
+__device__ __forceinline__ void kep3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,
+                                     const T n[3], T area, T F[5]) {
+  const T rho = T(0.5) * (l.rho + r.rho);
+  const T p = T(0.5) * (l.p + r.p);
+  T vel[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) vel[d] = l.u[d] * T(0.5) + r.u[d] * T(0.5);
+  const T al = m_sqrt(P.gamma * l.p / l.rho), ar = m_sqrt(P.gamma * r.p / r.rho);
+  const T Hl = al * al / (P.gamma - T(1)) + T(0.5) * vdot3(l.u, l.u);
+  const T Hr = ar * ar / (P.gamma - T(1)) + T(0.5) * vdot3(r.u, r.u);
+  const T H = T(0.5) * (Hl + Hr);
+  const T vn = vdot3(vel, n);
+  const T mass = rho * vn;
+  T mom[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * p + vel[d] * mass;
+  const T en = rho * H * vn;
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+template <typename T, int G>
+struct Euler3Ph {
+  using TS = T;
+  using TF = T;
+  using TP = T;
+  static constexpr int NV = 5, NREC = E3_NREC;
+
+  __device__ static Pr3<T> prim(const T* ld, int NB, int c) {
+    Pr3<T> q;
+    q.rho = ld[E3_RHO * NB + c];
+    q.u[0] = ld[E3_U * NB + c];
+    q.u[1] = ld[E3_V * NB + c];
+    q.u[2] = ld[E3_W * NB + c];
+    q.p = ld[E3_P * NB + c];
+    q.bk = ld[E3_BK * NB + c];
+    return q;
+  }
+
+  // con_to_prim fv.cc:388-400, wall mirror of the normal velocity, then
+  // prim_to_ent fv.cc:421-431 (Entropy 557-560)
+  __device__ static void load(const Phys<T>& P, const T* U, int64_t N, int64_t idx, int flip_axis,
+                              T* ld, int NB, int c) {
+    const T rho = U[idx], E = U[4 * N + idx];
+    const T inv = T(1) / rho;
+    T u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = U[(1 + d) * N + idx] * inv;
+    const T m2 = U[N + idx] * U[N + idx] + U[2 * N + idx] * U[2 * N + idx] +
+                 U[3 * N + idx] * U[3 * N + idx];
+    const T p = (P.gamma - T(1)) * (E - T(0.5) * m2 / rho);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (flip_axis & (1 << d)) u[d] = -u[d];
+    const T beta = rho / (T(2) * p);
+    const T s = m_log(p) - P.gamma * m_log(rho);
+    ld[E3_RHO * NB + c] = rho;
+    ld[E3_U * NB + c] = u[0];
+    ld[E3_V * NB + c] = u[1];
+    ld[E3_W * NB + c] = u[2];
+    ld[E3_P * NB + c] = p;
+    ld[E3_BK * NB + c] = T(0.5) / (P.R * (p / (P.R * rho)));
+    ld[E3_VF * NB + c] = (P.gamma - s) / (P.gamma - T(1)) - vdot3(u, u) * beta;
+    ld[E3_V1 * NB + c] = u[0] * (T(2) * beta);
+    ld[E3_V2 * NB + c] = u[1] * (T(2) * beta);
+    ld[E3_V3 * NB + c] = u[2] * (T(2) * beta);
+    ld[E3_VL * NB + c] = T(-2) * beta;
+  }
+
+  // FV::num_flux for one face from the 2G cells base + l*stride
+  __device__ static void face(const Phys<T>& P, const T* ld, int NB, int base, int stride, int ax,
+                              T F[5]) {
+    constexpr int S = 2 * G;
+    const int L = G - 1, R = G;
+    T n[3] = {T(0), T(0), T(0)};
+    n[ax] = T(1);
+    const T area = P.area[ax];
+    Pr3<T> q[S];
+#pragma unroll
+    for (int l = 0; l < S; ++l) q[l] = prim(ld, NB, base + l * stride);
+    const int fl = P.flux;
+    auto ec = [&](int kind, const Pr3<T>& a, const Pr3<T>& b, T out[5]) {
+      if (kind == 0)
+        kepec3(P, a, b, n, area, out);
+      else if (kind == 1)
+        roeec3(P, a, b, n, area, out);
+      else
+        kep3(P, a, b, n, area, out);
+    };
+    auto ec4 = [&](int kind, T out[5]) {  // kepec4.cc:10-38, roe_ec4.cc:10-34
+      T f0[5];
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v] = T(0);
+      ec(kind, q[L], q[R], f0);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v] += f0[v] * T(4.0 / 3.0);
+      ec(kind, q[L - 1], q[R], f0);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v] += f0[v] * T(-1.0 / 6.0);
+      ec(kind, q[L], q[R + 1], f0);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) out[v] += f0[v] * T(-1.0 / 6.0);
+    };
+    // pure entropy-conservative fluxes
+    if (fl == 120) { ec(0, q[L], q[R], F); return; }
+    if (fl == 121) { ec4(0, F); return; }
+    if (fl == 122) { ec(1, q[L], q[R], F); return; }
+    if (fl == 123) { ec4(1, F); return; }
+    if (fl == 124) { ec(2, q[L], q[R], F); return; }
+    // TeCNO: entropy-conservative part
+    if (fl == 101 || fl == 103) ec4(0, F);
+    else if (fl == 100 || fl == 102) ec(0, q[L], q[R], F);
+    else if (fl == 105) ec4(1, F);
+    else if (fl == 104) ec(1, q[L], q[R], F);
+    else ec(2, q[L], q[R], F);  // 106 kep_tecno_roe
+    const T inv_area = T(1) / area;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] *= inv_area;
+    // means of the centre pair, eigensystem (kepes_tecno4_roe.cc:39-113)
+    const Pr3<T>& pl = q[L];
+    const Pr3<T>& pr = q[R];
+    const T rho = logavg(pl.rho, pr.rho);
+    const T beta = logavg(pl.bk, pr.bk);
+    const T p = T(0.5) * (pl.rho + pr.rho) / (pl.bk + pr.bk);
+    T vel[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vel[d] = pl.u[d] * T(0.5) + pr.u[d] * T(0.5);
+    const T vn = vdot3(vel, n);
+    const T a = m_sqrt(T(0.5) * P.gamma / beta);
+    const T H = a * a / (P.gamma - T(1)) + T(0.5) * vdot3(vel, vel);
+    const T v1 = vel[0] * n[1] - vel[1] * n[0];
+    const T v2 = vel[2] * n[0] - vel[0] * n[2];
+    T Rm[5][5] = {{T(1), T(1), T(0), T(0), T(1)},
+                  {vel[0] - a * n[0], vel[0], n[1], -n[2], vel[0] + a * n[0]},
+                  {vel[1] - a * n[1], vel[1], -n[0], T(0), vel[1] + a * n[1]},
+                  {vel[2] - a * n[2], vel[2], T(0), n[0], vel[2] + a * n[2]},
+                  {H - a * vn, T(0.5) * vdot3(vel, vel), v1, v2, H + a * vn}};
+    const T const1 = m_sqrt(rho / P.gamma);
+    const T psq = m_sqrt(p);
+    const T Sc[5] = {m_sqrt(T(0.5)) * const1, m_sqrt(P.gamma - T(1)) * const1, psq, psq,
+                     m_sqrt(T(0.5)) * const1};
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+      for (int j = 0; j < 5; ++j) Rm[j][i] *= Sc[i];
+    T Lam[5];
+    if (fl == 102 || fl == 103) {
+      const T lr = m_abs(vn) + m_abs(a);
+#pragma unroll
+      for (int i = 0; i < 5; ++i) Lam[i] = lr;
+    } else {
+      const T l2 = m_abs(vn);
+      T ea = T(0);
+      if (P.ec1_fix && fl != 106) ea = T(1.0 / 6.0);
+      const T vnl = vdot3(pl.u, n), vnr = vdot3(pr.u, n);
+      const T al = m_sqrt(T(0.5) * P.gamma / pl.bk), ar = m_sqrt(T(0.5) * P.gamma / pr.bk);
+      Lam[0] = (T(1) + T(0)) * m_abs(vn - a) + ea * m_abs((vnl - al) - (vnr - ar));
+      Lam[1] = l2;
+      Lam[2] = l2;
+      Lam[3] = l2;
+      Lam[4] = (T(1) + T(0)) * m_abs(vn + a) + ea * m_abs(vnl - vnr);
+    }
+    // Z = R~^T V on the stencil, reconstruct, Diff = R~ Lambda dZ
+    T dZ[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      T st[S];
+#pragma unroll
+      for (int k = 0; k < S; ++k) {
+        const int cc = base + k * stride;
+        st[k] = T(0) + (Rm[0][i] * ld[E3_VF * NB + cc] + Rm[1][i] * ld[E3_V1 * NB + cc] +
+                        Rm[2][i] * ld[E3_V2 * NB + cc] + Rm[3][i] * ld[E3_V3 * NB + cc] +
+                        Rm[4][i] * ld[E3_VL * NB + cc]);
+      }
+      T zl, zr;
+      recon_any(P, st, G, zl, zr);
+      dZ[i] = zr - zl;
+    }
+    T Diff[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      Diff[i] = T(0);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) Diff[i] += Rm[i][j] * Lam[j] * dZ[j];
+    }
+    F[0] -= T(0.5) * Diff[0];
+    F[1] -= T(0.5) * Diff[1];
+    F[2] -= T(0.5) * Diff[2];
+    if (P.fix_zmom)
+      F[3] -= T(0.5) * Diff[3];
+    else
+      F[2] -= T(0.5) * Diff[3];  // kepes_tecno4_roe.cc:170 (z dissipation into y)
+    F[4] -= T(0.5) * Diff[4];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) F[v] *= area;
+  }
+
+  // viscous node fluxes (momentum x, y, z and energy of the x, y and z
+  // fluxes, already times -area) from the 8 cells c[a][b][q] around a node
+  __device__ static void node(const Phys<T>& P, const T* ld, int NB, const int c[2][2][2],
+                              T X[4], T Y[4], T Z[4]) {
+    const T dx = P.h[0], dy = P.h[1], dz = P.h[2];
+    const T f1 = T(1.0 / 8.0);
+    if (P.model == 12) {
+      // nodal_vlux_ES vflux_ES.cc:15-151: node averages and differences of V
+      const T fx = T(1.0 / 4.0) / dx, fy = T(1.0 / 4.0) / dy, fz = T(1.0 / 4.0) / dz;
+      T avg[5], Tx[5], Ty[5], Tz[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int r = E3_VF + k;
+        auto V = [&](int a, int b, int q) { return ld[r * NB + c[a][b][q]]; };
+        avg[k] = V(1, 1, 1) * f1 + V(0, 1, 1) * f1 + V(1, 0, 1) * f1 + V(1, 1, 0) * f1;
+        avg[k] += V(0, 0, 1) * f1 + V(0, 1, 0) * f1 + V(1, 0, 0) * f1 + V(0, 0, 0) * f1;
+        Tx[k] = V(1, 1, 1) * fx + V(1, 1, 0) * fx + V(1, 0, 1) * fx + V(1, 0, 0) * fx;
+        Tx[k] += V(0, 1, 1) * -fx + V(0, 1, 0) * -fx + V(0, 0, 1) * -fx + V(0, 0, 0) * -fx;
+        Ty[k] = V(1, 1, 1) * fy + V(1, 1, 0) * fy + V(0, 1, 1) * fy + V(0, 1, 0) * fy;
+        Ty[k] += V(1, 0, 1) * -fy + V(1, 0, 0) * -fy + V(0, 0, 1) * -fy + V(0, 0, 0) * -fy;
+        Tz[k] = V(1, 1, 1) * fz + V(1, 0, 1) * fz + V(0, 1, 1) * fz + V(0, 0, 1) * fz;
+        Tz[k] += V(1, 1, 0) * -fz + V(1, 0, 0) * -fz + V(0, 1, 0) * -fz + V(0, 0, 0) * -fz;
+      }
+      const T T_avg = T(-1) / avg[4] / P.R;
+      const T mu = T(visc_mu(P, (double)T_avg));
+      const T kappa = mu * T(P.Cp) / T(P.prandtl);
+      const T c1 = T(4) * mu / T(3), c2 = T(-2) * mu / T(3), c3 = kappa / P.R, e4 = T(1) / avg[4],
+              e1 = avg[1] * e4, e2 = avg[2] * e4, e3 = avg[3] * e4;
+      // x flux: K11 V_x + K12 V_y + K13 V_z
+      X[0] = c1 * e4 * (-Tx[1] + e1 * Tx[4]);
+      X[1] = mu * e4 * (-Tx[2] + e2 * Tx[4]);
+      X[2] = mu * e4 * (-Tx[3] + e3 * Tx[4]);
+      X[3] = e4 * (c1 * e1 * Tx[1] + mu * e2 * Tx[2] + mu * e3 * Tx[3] +
+                   (-c1 * e1 * e1 - mu * (e2 * e2 + e3 * e3) + c3 * e4) * Tx[4]);
+      X[0] += c2 * e4 * (-Ty[2] + e2 * Ty[4]);
+      X[1] += mu * e4 * (-Ty[1] + e1 * Ty[4]);
+      X[3] += e4 * (mu * e2 * Ty[1] + c2 * e1 * Ty[2] - (mu + c2) * e1 * e2 * Ty[4]);
+      X[0] += c2 * e4 * (-Tz[3] + e3 * Tz[4]);
+      X[2] += mu * e4 * (-Tz[1] + e1 * Tz[4]);
+      X[3] += e4 * (mu * e3 * Tz[1] + c2 * e1 * Tz[3] - (mu + c2) * e1 * e3 * Tz[4]);
+      // y flux: the reference's K21 part is overwritten by the K22 assignment
+      Y[0] = mu * e4 * (-Ty[1] + e1 * Ty[4]);
+      Y[1] = c1 * e4 * (-Ty[2] + e2 * Ty[4]);
+      Y[2] = mu * e4 * (-Ty[3] + e3 * Ty[4]);
+      Y[3] = e4 * (mu * e1 * Ty[1] + c1 * e2 * Ty[2] + mu * e3 * Ty[3] +
+                   (-c1 * e2 * e2 - mu * (e1 * e1 + e3 * e3) + c3 * e4) * Ty[4]);
+      Y[1] += c2 * e4 * (-Tz[3] + e3 * Tz[4]);
+      Y[2] += mu * e4 * (-Tz[2] + e2 * Tz[4]);
+      Y[3] += e4 * (mu * e3 * Tz[2] + c2 * e2 * Tz[3] - (mu + c2) * e2 * e3 * Tz[4]);
+      // z flux: K31 and K32 parts overwritten by the K33 assignment
+      Z[0] = mu * e4 * (-Tz[1] + e1 * Tz[4]);
+      Z[1] = mu * e4 * (-Tz[2] + e2 * Tz[4]);
+      Z[2] = c1 * e4 * (-Tz[3] + e3 * Tz[4]);
+      Z[3] = e4 * (mu * e1 * Tz[1] + mu * e2 * Tz[2] + c1 * e3 * Tz[3] +
+                   (-c1 * e3 * e3 - mu * (e1 * e1 + e2 * e2) + c3 * e4) * Tz[4]);
+    } else {
+      // nodal_vlux_KE vflux_KE.cc:15-96 with nodal_avg_and_vel_derivatives
+      const T fx = T(1) / (T(4) * dx), fy = T(1) / (T(4) * dy), fz = T(1) / (T(4) * dz);
+      auto Q = [&](int r, int a, int b, int q) { return ld[r * NB + c[a][b][q]]; };
+      T avg[5];
+      const int rr[5] = {E3_RHO, E3_U, E3_V, E3_W, E3_P};
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int r = rr[k];
+        avg[k] = Q(r, 1, 1, 1) * f1 + Q(r, 0, 1, 1) * f1 + Q(r, 1, 0, 1) * f1 + Q(r, 1, 1, 0) * f1;
+        avg[k] += Q(r, 0, 0, 1) * f1 + Q(r, 0, 1, 0) * f1 + Q(r, 1, 0, 0) * f1 + Q(r, 0, 0, 0) * f1;
+      }
+      T Tx[3], Ty[3], Tz[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int r = E3_U + d;
+        Tx[d] = Q(r, 1, 1, 1) * fx + Q(r, 1, 1, 0) * fx + Q(r, 1, 0, 1) * fx + Q(r, 1, 0, 0) * fx;
+        Tx[d] += Q(r, 0, 1, 1) * -fx + Q(r, 0, 1, 0) * -fx + Q(r, 0, 0, 1) * -fx + Q(r, 0, 0, 0) * -fx;
+        Ty[d] = Q(r, 1, 1, 1) * fy + Q(r, 1, 1, 0) * fy + Q(r, 0, 1, 1) * fy + Q(r, 0, 1, 0) * fy;
+        Ty[d] += Q(r, 1, 0, 1) * -fy + Q(r, 1, 0, 0) * -fy + Q(r, 0, 0, 1) * -fy + Q(r, 0, 0, 0) * -fy;
+        Tz[d] = Q(r, 1, 1, 1) * fz + Q(r, 1, 0, 1) * fz + Q(r, 0, 1, 1) * fz + Q(r, 0, 0, 1) * fz;
+        Tz[d] += Q(r, 1, 1, 0) * -fz + Q(r, 1, 0, 0) * -fz + Q(r, 0, 1, 0) * -fz + Q(r, 0, 0, 0) * -fz;
+      }
+      auto Tc = [&](int a, int b, int q) { return Q(E3_P, a, b, q) / (P.R * Q(E3_RHO, a, b, q)); };
+      const T T_avg = avg[4] / (P.R * avg[0]);
+      const T mu = T(visc_mu(P, (double)T_avg));
+      const T kappa = mu * T(P.Cp) / T(P.prandtl);
+      const T Txt = fx * (Tc(1, 1, 1) + Tc(1, 1, 0) + Tc(1, 0, 1) + Tc(1, 0, 0) - Tc(0, 1, 1) -
+                          Tc(0, 1, 0) - Tc(0, 0, 1) - Tc(0, 0, 0));
+      const T Tyt = fy * (Tc(1, 1, 1) + Tc(1, 1, 0) + Tc(0, 1, 1) + Tc(0, 1, 0) - Tc(1, 0, 1) -
+                          Tc(1, 0, 0) - Tc(0, 0, 1) - Tc(0, 0, 0));
+      const T Tzt = fz * (Tc(1, 1, 1) + Tc(1, 0, 1) + Tc(0, 1, 1) + Tc(0, 0, 1) - Tc(1, 1, 0) -
+                          Tc(1, 0, 0) - Tc(0, 1, 0) - Tc(0, 0, 0));
+      const T gv = Tx[0] + Ty[1] + Tz[2];
+      const T txx = mu * (T(2) * Tx[0] - (T(2) / T(3)) * gv);
+      const T tyy = mu * (T(2) * Ty[1] - (T(2) / T(3)) * gv);
+      const T tzz = mu * (T(2) * Tz[2] - (T(2) / T(3)) * gv);
+      const T txy = mu * (Ty[0] + Tx[1]), txz = mu * (Tx[2] + Tz[0]), tyz = mu * (Ty[2] + Tz[1]);
+      const T* u = &avg[1];
+      X[0] = txx; X[1] = txy; X[2] = txz;
+      X[3] = u[0] * txx + u[1] * txy + u[2] * txz + kappa * Txt;
+      Y[0] = txy; Y[1] = tyy; Y[2] = tyz;
+      Y[3] = u[0] * txy + u[1] * tyy + u[2] * tyz + kappa * Tyt;
+      Z[0] = txz; Z[1] = tyz; Z[2] = tzz;
+      Z[3] = u[0] * txz + u[1] * tyz + u[2] * tzz + kappa * Tzt;
+    }
+    const T ax_ = -(dy * dz), ay_ = -(dx * dz), az_ = -(dx * dy);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      X[v] *= ax_;
+      Y[v] *= ay_;
+      Z[v] *= az_;
+    }
+  }
+
+  // compute_dt fv.cc:366-383 (viscous term for the NS models)
+  __device__ static T speed(const Phys<T>& P, const T* U, int64_t N, int64_t idx) {
+    const T rho = U[idx], E = U[4 * N + idx];
+    const T inv = T(1) / rho;
+    T u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = U[(1 + d) * N + idx] * inv;
+    const T m2 = U[N + idx] * U[N + idx] + U[2 * N + idx] * U[2 * N + idx] +
+                 U[3 * N + idx] * U[3 * N + idx];
+    const T p = (P.gamma - T(1)) * (E - T(0.5) * m2 / rho);
+    T s = m_sqrt(vdot3(u, u)) + m_sqrt(P.gamma * p / rho);
+    if (P.model == 11 || P.model == 12)
+      s += T(visc_mu(P, (double)(p / (P.R * rho)))) / rho / P.hmin;
+    return s;
+  }
+  __device__ static bool bad(const T* U, int64_t N, int64_t idx) {
+    const T rho = U[idx];
+    const T m2 = U[N + idx] * U[N + idx] + U[2 * N + idx] * U[2 * N + idx] +
+                 U[3 * N + idx] * U[3 * N + idx];
+    const T e = U[4 * N + idx] - T(0.5) * m2 / rho;
+    return !(rho > T(0)) || !(e > T(0)) || !(e < T(INFINITY));
+  }
+};
+
+// ================================================================ 3D tile
+template <class PH, int TX, int TY, int TZ, int G, bool VISC>
+struct Tile3 {
+  using T = typename PH::TS;
+  static constexpr int BX = TX + 2 * G, BY = TY + 2 * G, BZ = TZ + 2 * G;
+  static constexpr int NB = BX * BY * BZ;
+  static constexpr int NFX = (TX + 1) * TY * TZ, NFY = TX * (TY + 1) * TZ;
+  static constexpr int NFZ = TX * TY * (TZ + 1), NF = NFX + NFY + NFZ;
+  static constexpr int ROUNDS = (NF + 255) / 256;
+  static constexpr int NN = (TX + 1) * (TY + 1) * (TZ + 1);
+  static constexpr int NCELL = TX * TY * TZ;
+  static constexpr int NV = PH::NV;
+  static constexpr size_t REC_BYTES = (size_t)PH::NREC * NB * sizeof(T);
+  static constexpr size_t FLUX_BYTES = (size_t)NV * NF * sizeof(T);
+  static constexpr size_t NODE_BYTES = VISC ? (size_t)12 * NN * sizeof(T) : 0;
+  static constexpr size_t LDS_BYTES =
+      REC_BYTES > FLUX_BYTES + NODE_BYTES ? REC_BYTES : FLUX_BYTES + NODE_BYTES;
+  static_assert(NN <= 256, "one node per lane");
+
+  __device__ static int box(int x, int y, int z) { return (z * BY + y) * BX + x; }
+
+  __device__ static void load(const StageArgs<T>& A, const Phys<T>& P, const GridDev& g, T* ld,
+                              int tid, int x0, int y0, int z0) {
+    for (int c = tid; c < NB; c += 256) {
+      const int bx = c % BX, by = (c / BX) % BY, bz = c / (BX * BY);
+      const int lx = bx - G, ly = by - G, lz = bz - G;
+      const int out = (lx < 0 || lx >= TX) + (ly < 0 || ly >= TY) + (lz < 0 || lz >= TZ);
+      if (out > (VISC ? 3 : 1)) continue;  // outside every face stencil
+      int fx, fy, fz;
+      const int sx = bc_map(x0 + lx, g.n[0], g.bc[0], g.bc[1], fx);
+      const int sy = bc_map(y0 + ly, g.n[1], g.bc[2], g.bc[3], fy);
+      const int sz = bc_map(z0 + lz, g.n[2], g.bc[4], g.bc[5], fz);
+      const int64_t idx = ((int64_t)sz * g.n[1] + sy) * g.n[0] + sx;
+      PH::load(P, A.U, g.N, idx, fx | (fy << 1) | (fz << 2), ld, NB, c);
+    }
+  }
+
+  __device__ static void faces(const Phys<T>& P, const T* ld, int tid, T Fr[ROUNDS][NV]) {
+#pragma unroll
+    for (int rd = 0; rd < ROUNDS; ++rd) {
+      int f = rd * 256 + tid;
+      if (rd * 256 + (tid & ~63) >= NF) break;  // whole wave past the end
+      f = f < NF ? f : NF - 1;
+      int base, stride, ax;
+      if (f < NFX) {
+        const int x = f % (TX + 1), y = (f / (TX + 1)) % TY, z = f / ((TX + 1) * TY);
+        base = box(x, y + G, z + G);  // cells x .. x + 2G - 1 along x
+        stride = 1;
+        ax = 0;
+      } else if (f < NFX + NFY) {
+        const int q = f - NFX;
+        const int x = q % TX, y = (q / TX) % (TY + 1), z = q / (TX * (TY + 1));
+        base = box(x + G, y, z + G);
+        stride = BX;
+        ax = 1;
+      } else {
+        const int q = f - NFX - NFY;
+        const int x = q % TX, y = (q / TX) % TY, z = q / (TX * TY);
+        base = box(x + G, y + G, z);
+        stride = BX * BY;
+        ax = 2;
+      }
+      PH::face(P, ld, NB, base, stride, ax, Fr[rd]);
+    }
+  }
+
+  // node (a, b, q) of the tile lies between cells a-1 and a (local) along x;
+  // nodes on two global boundaries count twice, as in compute_visc_residual
+  __device__ static void node(const Phys<T>& P, const GridDev& g, const T* ld, int tid, int x0,
+                              int y0, int z0, T X[4], T Y[4], T Z[4]) {
+    const int nx = tid % (TX + 1), ny = (tid / (TX + 1)) % (TY + 1), nz = tid / ((TX + 1) * (TY + 1));
+    int c[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) c[a][b][q] = box(nx - 1 + a + G, ny - 1 + b + G, nz - 1 + q + G);
+    PH::node(P, ld, NB, c, X, Y, Z);
+    const int gx = x0 + nx - 1, gy = y0 + ny - 1, gz = z0 + nz - 1;
+    const int nb = (gx == -1 || gx == g.n[0] - 1) + (gy == -1 || gy == g.n[1] - 1) +
+                   (gz == -1 || gz == g.n[2] - 1);
+    if (nb == 2) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        X[v] *= T(2);
+        Y[v] *= T(2);
+        Z[v] *= T(2);
+      }
+    }
+  }
+
+  __device__ static void update(const StageArgs<T>& A, const Phys<T>& P, const GridDev& g,
+                                const T* lf, const T* lnode, int tid, int x0, int y0, int z0,
+                                double dt, T& smax, unsigned& nbad) {
+    const int64_t N = g.N;
+    const double factor = dt / P.vol_d;
+    const T a = T(A.a), b = T(A.b), fb = T(factor * A.b);
+    for (int c = tid; c < NCELL; c += 256) {
+      const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
+      const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+      if (gx >= g.n[0] || gy >= g.n[1] || gz >= g.n[2]) continue;
+      const int64_t idx = ((int64_t)gz * g.n[1] + gy) * g.n[0] + gx;
+      const int fxl = (lz * TY + ly) * (TX + 1) + lx;
+      const int fyl = NFX + (lz * (TY + 1) + ly) * TX + lx;
+      const int fzl = NFX + NFY + (lz * TY + ly) * TX + lx;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        T r = -lf[v * NF + fxl];
+        r += lf[v * NF + fxl + 1];
+        r -= lf[v * NF + fyl];
+        r += lf[v * NF + fyl + TX];
+        r -= lf[v * NF + fzl];
+        r += lf[v * NF + fzl + TX * TY];
+        if (VISC && v > 0) {
+          // the 8 corner nodes: (+1/4) on the upper side of a direction
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+              for (int aa = 0; aa < 2; ++aa) {
+                const int nd = ((lz + q) * (TY + 1) + (ly + bb)) * (TX + 1) + (lx + aa);
+                const T sx = aa ? T(0.25) : T(-0.25), sy = bb ? T(0.25) : T(-0.25),
+                        sz = q ? T(0.25) : T(-0.25);
+                r += lnode[(v - 1) * NN + nd] * sx + lnode[(4 + v - 1) * NN + nd] * sy +
+                     lnode[(8 + v - 1) * NN + nd] * sz;
+              }
+        }
+        const T u = A.U[v * N + idx];
+        const T u0 = A.U0[v * N + idx];
+        T un = u * b;
+        un += (u0 * a - fb * r);
+        A.Uout[v * N + idx] = un;
+      }
+      if (A.last) {
+        smax = m_max(smax, PH::speed(P, A.Uout, N, idx));
+        nbad += PH::bad(A.Uout, N, idx) ? 1u : 0u;
+      }
+    }
+  }
+};
+
+#ifndef TECNO_HOST_BUILD
+template <class PH, int TX, int TY, int TZ, int G, bool VISC>
+__global__ __launch_bounds__(256) void stage3_kernel(StageArgs<typename PH::TS> A,
+                                                     Phys<typename PH::TP> P, GridDev g) {
+  using TL = Tile3<PH, TX, TY, TZ, G, VISC>;
+  using T = typename PH::TS;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[TL::LDS_BYTES];
+  __shared__ double red[4];
+  if (!A.ctl->active) return;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * TZ;
+  T* ld = (T*)lds;
+  TL::load(A, P, g, ld, tid, x0, y0, z0);
+  __syncthreads();
+  T Fr[TL::ROUNDS][TL::NV];
+  TL::faces(P, ld, tid, Fr);
+  T X[4], Y[4], Z[4];
+  if (VISC && tid < TL::NN) TL::node(P, g, ld, tid, x0, y0, z0, X, Y, Z);
+  __syncthreads();
+  T* lf = ld;
+  T* lnode = ld + (size_t)TL::NV * TL::NF;
+#pragma unroll
+  for (int rd = 0; rd < TL::ROUNDS; ++rd) {
+    const int f = rd * 256 + tid;
+    if (f < TL::NF)
+#pragma unroll
+      for (int v = 0; v < TL::NV; ++v) lf[v * TL::NF + f] = Fr[rd][v];
+  }
+  if (VISC && tid < TL::NN)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      lnode[v * TL::NN + tid] = X[v];
+      lnode[(4 + v) * TL::NN + tid] = Y[v];
+      lnode[(8 + v) * TL::NN + tid] = Z[v];
+    }
+  __syncthreads();
+  T smax = T(0);
+  unsigned nbad = 0;
+  TL::update(A, P, g, lf, lnode, tid, x0, y0, z0, A.ctl->dt, smax, nbad);
+  if (A.last) {
+    double s = (double)smax;
+    for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off));
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+      double m = red[0];
+      for (int w = 1; w < 4; ++w) m = fmax(m, red[w]);
+      atomic_max_pos(&A.ctl->smax_bits, m);
+    }
+    if (nbad) atomicAdd(&A.ctl->nbad, (unsigned long long)nbad);
+  }
+}
+#endif
+
+}  // namespace tecno

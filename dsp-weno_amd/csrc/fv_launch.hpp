@@ -5,6 +5,7 @@
 #pragma once
 #include "fv_kernels.hpp"
 #include "fv_mixed.hpp"
+#include "fv_euler3.hpp"
 
 namespace tecno {
 
@@ -383,6 +384,19 @@ static hipError_t launch_stage_t(const LaunchInfo& L, const Phys<typename PH::TP
   return hipGetLastError();
 }
 
+template <class PH, int G, bool VISC>
+static hipError_t launch_stage3_t(const LaunchInfo& L, const Phys<typename PH::TP>& P,
+                                  const void* U, const void* U0, void* Uo, double a, double b,
+                                  int last) {
+  using TS = typename PH::TS;
+  constexpr int TX = 8, TY = 8, TZ = 2;
+  StageArgs<TS> A{(const TS*)U, (const TS*)U0, (TS*)Uo, L.ctl, a, b, last};
+  dim3 grid((L.grid.n[0] + TX - 1) / TX, (L.grid.n[1] + TY - 1) / TY, (L.grid.n[2] + TZ - 1) / TZ);
+  hipLaunchKernelGGL((stage3_kernel<PH, TX, TY, TZ, G, VISC>), grid, dim3(256), 0, L.stream, A, P,
+                     L.grid);
+  return hipGetLastError();
+}
+
 template <class PH>
 static hipError_t launch_speed_t(const LaunchInfo& L, const Phys<typename PH::TP>& P,
                                  const void* U) {
@@ -404,6 +418,16 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
     if (L.dims == 2)
       return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 2, X2, Y2, 3>(L, P, U, U0, Uo, a, b, last)
                       : launch_stage_t<ScalarPh<T, 2>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
+    return hipErrorNotSupported;
+  }
+  if (L.dims == 3) {
+    // 3D Euler / Navier-Stokes: FP64 reference arithmetic only
+    if constexpr (sizeof(T) == 8) {
+      const bool visc = P.model == 11 || P.model == 12;
+      if (L.G != 2) return hipErrorNotSupported;
+      if (visc) return launch_stage3_t<Euler3Ph<T, 2>, 2, true>(L, P, U, U0, Uo, a, b, last);
+      return launch_stage3_t<Euler3Ph<T, 2>, 2, false>(L, P, U, U0, Uo, a, b, last);
+    }
     return hipErrorNotSupported;
   }
   if (L.dims != 2) return hipErrorNotSupported;
@@ -437,6 +461,10 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
 template <typename T>
 hipError_t Launch<T>::speed(const LaunchInfo& L, const Phys<T>& P, const void* U) {
   if (L.system == 0) return launch_speed_t<ScalarPh<T, 2>>(L, P, U);
+  if (L.dims == 3) {
+    if constexpr (sizeof(T) == 8) return launch_speed_t<Euler3Ph<T, 2>>(L, P, U);
+    return hipErrorNotSupported;
+  }
   if constexpr (sizeof(T) == 4)
     return launch_speed_t<Euler2Mixed<RV_SPWENO, MM_SIMT>>(L, P, U);
   else

@@ -77,6 +77,43 @@ def test_euler2d_golden_fp64(key):
         assert rel_err(got, ref) <= 1e-12 * int(k), (key, k, rel_err(got, ref))
 
 
+EULER3_KEYS = sorted({k.split("/")[0] for k in fixtures("euler").files
+                      if k.startswith("eu3d") or k.startswith("ns3d")})
+
+
+@pytest.mark.parametrize("key", EULER3_KEYS)
+def test_euler3d_ns_golden_fp64(key):
+    """3D Euler and Navier-Stokes (ES / KE viscous fluxes, constant,
+    Sutherland and power-law viscosity, sp_weno / sp_weno_dl) Taylor-Green
+    runs of the reference: 1e-12 relative per step."""
+    d = case("euler", key)
+    cfg, final, stamps = euler_cfg(key)
+    s = solver_for(cfg, key.endswith("dsp"))
+    s.set_state(np.asarray(d["init"]).reshape(5, -1))
+    for i, k in enumerate(d["steps"]):
+        s.advance(final, stamps, max_iter=int(k))
+        ref = np.asarray(d["states"][i]).reshape(5, -1)
+        got = s.get_state()
+        assert rel_err(got, ref) <= 1e-12 * int(k), (key, k, rel_err(got, ref))
+
+
+def test_ns3d_per_step_vs_oracle_fix_zmom():
+    """3D Navier-Stokes with the z-momentum dissipation fix (no reference run
+    exists for it): GPU against the oracle, 3 steps, 1e-12 relative."""
+    cfg, final, stamps = euler_cfg("ns3d_es_tg", fix_zmom=1)
+    d = case("euler", "ns3d_es_tg")
+    init = np.asarray(d["init"]).reshape(5, -1)
+    o = orc.OracleSolver(cfg, None, faithful=True)
+    o.set_state(init)
+    s = Solver(cfg)
+    s.set_state(init)
+    for k in range(3):
+        o.step(1e300)
+        s.step(1e300)
+        ref = o.get_state()
+        assert rel_err(s.get_state(), ref) <= 1e-12 * (k + 1)
+
+
 def test_config1_l1_gpu():
     """Config 1: lin_tecno4 + sp_weno, Nx = 400, t = 0.5 -> L1 = 1.75268e-05
     (the reference's own figure), 65 steps."""

@@ -1,0 +1,161 @@
+"""`tecno -i param.in` on the CPU (no GPU is touched with --dry-run): the
+param.in reader and the expression evaluator against the reference's own
+initial states (the golden fixtures hold them), the error paths, and
+operator semantics."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from cases import fixtures
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "dsp-weno_amd", "tecno")
+CASES = os.path.join(ROOT, "tests", "golden", "cases")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(CLI), reason="tecno CLI not built")
+
+
+def dry(path):
+    r = subprocess.run([CLI, "-i", path, "--dry-run"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = {}
+    ic = {}
+    for line in r.stdout.splitlines():
+        k, *v = line.split()
+        if k == "ic":
+            ic[int(v[0])] = (float(v[1]), float(v[2]))
+        else:
+            out[k] = v
+    return out, ic
+
+
+@pytest.mark.parametrize("case,fx,key", [
+    ("config1_linadv_spweno.param.in", "scalar", "config1_spweno"),
+    ("kh_spweno_24.param.in", "euler", "eu2d_kh_spweno"),
+    ("kh_dsp_24.param.in", "euler", "eu2d_kh_dsp")])
+def test_initial_condition_matches_reference(case, fx, key):
+    """sums and sums of squares of every IC field over the reference's cells
+    (duplicate layers included) equal those of the reference's initial state"""
+    out, ic = dry(os.path.join(CASES, case))
+    init = np.asarray(fixtures(fx)[f"{key}/init"])
+    assert len(ic) == init.shape[0]
+    for v in range(init.shape[0]):
+        s1, s2 = ic[v]
+        r1, r2 = init[v].sum(), (init[v] ** 2).sum()
+        assert abs(s1 - r1) <= 1e-12 * max(1.0, np.abs(init[v]).sum())
+        assert abs(s2 - r2) <= 1e-12 * max(1.0, r2)
+
+
+def test_parsed_fields():
+    out, _ = dry(os.path.join(CASES, "kh_dsp_24.param.in"))
+    assert out["system"] == ["euler"] and out["flux"] == ["101"] and out["recon"][0] == "2"
+    assert out["reconstruction"] == ["sp_weno_dl"] and out["t_stamps"] == ["5"]
+    out, _ = dry(os.path.join(CASES, "vortex_spweno_ooc.param.in"))
+    assert out["levels"] == ["2"] and out["ooc"] == ["1"] and out["pdim"] == ["2"]
+    out, _ = dry(os.path.join(CASES, "config1_linadv_spweno.param.in"))
+    assert out["system"] == ["scalar"] and out["flux"] == ["3"] and out["exact"] == ["1"]
+
+
+SCALAR = """grid
+{{
+   x_min   {xmin}
+   x_max   {xmax}
+   Nx_cell {{ 1 }}
+   y_min   0.0
+   y_max   1.0
+   Ny_cell {{ 1 }}
+   z_min   0.0
+   z_max   1.0
+   Nz_cell {{ 1 }}
+}}
+numeric
+{{
+   time_scheme  {ts}
+   cfl          0.5
+   max_iter     10
+   final_time   0.1
+   min_residue  1.0e-6
+   reconstruct  {rec}
+   net_file     none
+}}
+material
+{{
+   model       linadv
+   vel         {{ 1 0 0 }}
+   flux        {flux}
+}}
+constants
+{{
+   a   2.5   # a comment
+   b   -1.0
+}}
+initial_condition
+{{
+   {expr}
+}}
+exact_soln
+{{
+   available   no
+}}
+boundary_condition
+{{
+   xleft {{ type {bc} }}
+   xright {{ type {bc} }}
+   yleft {{ type periodic }}
+   yright {{ type periodic }}
+   zleft {{ type periodic }}
+   zright {{ type periodic }}
+}}
+output
+{{
+   OOC_study   no
+   problem_dim 1
+   time_stamps 1
+   output_path .
+}}
+"""
+
+
+def scalar_value(expr, x=0.5, **kw):
+    args = dict(xmin=0.0, xmax=2 * x, ts="ssprk3", rec="sp_weno", flux="lin_tecno4", bc="periodic")
+    args.update(kw)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "param.in")
+        with open(p, "w") as f:
+            f.write(SCALAR.format(expr=expr, **args))
+        r = subprocess.run([CLI, "-i", p, "--dry-run"], capture_output=True, text=True, timeout=60)
+    return r
+
+
+@pytest.mark.parametrize("expr,value", [
+    ("-2^2", -4.0), ("2^-1", 0.5), ("2^3^2", 512.0), ("7%3", 1.0), ("(3>2)+(1=1)+(2!=2)", 2.0),
+    ("(x<1)&(x>0)", 1.0), ("(x>1)|(x<0)", 0.0), ("!0 + !3", 1.0), ("pow(a, 2)*b", -6.25),
+    ("if(x>0.4, 10, 20)", 10.0), ("abs(b)+sqrt(16)+exp(0)+log(1)", 6.0), ("min(a,b)+max(a,b)", 1.5),
+    ("pi*2/pi", 2.0), ("atan2(1,1)*4", np.pi), ("1e-2*100", 1.0), ("sin(x)^2+cos(x)^2", 1.0)])
+def test_expression_semantics(expr, value):
+    r = scalar_value(expr)
+    assert r.returncode == 0, r.stderr
+    line = [l for l in r.stdout.splitlines() if l.startswith("ic 0")][0]
+    assert abs(float(line.split()[2]) - value) <= 1e-14 * max(1.0, abs(value)), line
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(flux="keps4"), "unknown scalar flux"), (dict(ts="rk4"), "time_scheme"),
+    (dict(rec="minmod"), "reconstruct minmod"), (dict(bc="dirichlet"), "dirichlet"),
+    (dict(expr="sin(x"), "missing )"), (dict(expr="foo(x)"), "unknown function"),
+    (dict(expr="x + q"), "unknown name q")])
+def test_errors_are_reported(kw, msg):
+    args = dict(expr="x")
+    args.update(kw)
+    expr = args.pop("expr")
+    r = scalar_value(expr, **args)
+    assert r.returncode != 0 and msg in r.stderr, (r.returncode, r.stderr)
+
+
+def test_missing_file():
+    r = subprocess.run([CLI, "-i", "/nonexistent/param.in", "--dry-run"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode != 0 and "cannot open" in r.stderr

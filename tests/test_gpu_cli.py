@@ -1,0 +1,71 @@
+"""`tecno -i param.in` end to end on the GPU: the reference's OOC tables
+(config 1 figures from the reference run) and the solution / globals files."""
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from cases import euler_cfg, euler_prim_for, case
+from tecno import Solver
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "dsp-weno_amd", "tecno")
+CASES = os.path.join(ROOT, "tests", "golden", "cases")
+
+
+def run_cli(param, cwd):
+    r = subprocess.run([CLI, "-i", param, "--quiet"], capture_output=True, text=True, timeout=300,
+                       cwd=cwd)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def test_config1_ooc_table():
+    """lin_tecno4 + sp_weno, sin^4 on [-pi, pi], Nx = 100..800, t = 0.5: the
+    reference reports L1 = 1.75268e-05 at Nx = 400 and orders 3.17, 3.21, 3.25"""
+    text = open(os.path.join(CASES, "config1_linadv_spweno.param.in")).read()
+    text = text.replace("Nx_cell { 400 }", "Nx_cell { 100 200 400 800 }")
+    text = text.replace("Ny_cell { 2 }", "Ny_cell { 2 2 2 2 }").replace("Nz_cell { 2 }", "Nz_cell { 2 2 2 2 }")
+    text = text.replace("OOC_study   no", "OOC_study   yes")
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "param.in")
+        open(p, "w").write(text)
+        run_cli(p, d)
+        rows = open(os.path.join(d, "SOLN_FILES_sp_weno", "l1_err.dat")).read().splitlines()[3:]
+    toks = [r.split() for r in rows if r.strip()]
+    assert [t[0] for t in toks] == ["100", "200", "400", "800"]
+    assert toks[2][3] == "1.75268e-05"
+    orders = [float(t[4]) for t in toks[1:]]
+    assert [round(o, 2) for o in orders] == [3.17, 3.21, 3.25], orders
+
+
+def test_kh_files_and_state():
+    """KH 24x24 (x2 layers) to t = 1 with 5 output stamps: sol0000..sol0005.vtk,
+    one globals line per step, and the last file holds the state the library
+    reaches for the same run"""
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "param.in")
+        shutil.copy(os.path.join(CASES, "kh_spweno_24.param.in"), p)
+        run_cli(p, d)
+        out = os.path.join(d, "SOLN_FILES_sp_weno")
+        files = sorted(f for f in os.listdir(out) if f.startswith("sol"))
+        assert files == [f"sol{k:04d}.vtk" for k in range(6)]
+        glob = [l for l in open(os.path.join(out, "globals.dat")).read().splitlines() if l.strip()]
+        text = open(os.path.join(out, "sol0005.vtk")).read()
+    assert "DIMENSIONS 25 25 3" in text and "CELL_DATA  1152" in text
+    dens = text.split("SCALARS density float 1\nLOOKUP_TABLE default\n")[1].split("VECTORS")[0].split()
+    rho_file = np.array([float(v) for v in dens]).reshape(2, 24, 24)
+    assert np.array_equal(rho_file[0], rho_file[1])
+    cfg, final, stamps = euler_cfg("eu2d_kh_spweno")
+    s = Solver(cfg)
+    s.set_state(euler_prim_for(case("euler", "eu2d_kh_spweno")["init"], 2))
+    it, t = s.advance(final, stamps)
+    assert len(glob) == it
+    rho = s.get_state()[0].reshape(24, 24)
+    assert np.allclose(rho_file[0], rho, rtol=1e-5, atol=0)
+    assert float(glob[0].split()[0]) == 0.0

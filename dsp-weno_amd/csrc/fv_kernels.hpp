@@ -75,6 +75,23 @@ struct TileShape {
   static constexpr int NF = NFX + NFY;
   static constexpr int ROUNDS = (NF + 255) / 256;
   static constexpr int NCELL = TX * TY;
+  // distance-2 cell pairs (c, c+2) along each axis: the pairs (0,2) and (1,3)
+  // of the 4-cell stencil of a face are the pairs p and p+1 of its row
+  // (x faces) or column (y faces), so neighbouring faces share one
+  static constexpr int NPX = (TX + 2) * TY;
+  static constexpr int NPY = (DIMS >= 2) ? TX * (TY + 2) : 0;
+  static constexpr int NP = NPX + NPY;
+};
+
+// PH::PAIRS: the face flux takes the two-point fluxes of the distance-2 pairs
+// of its stencil from LDS (computed once per pair) instead of evaluating them
+template <class PH, class = void>
+struct has_pairs {
+  static constexpr bool value = false;
+};
+template <class PH>
+struct has_pairs<PH, decltype((void)PH::PAIRS)> {
+  static constexpr bool value = PH::PAIRS;
 };
 
 #ifndef TECNO_HOST_BUILD
@@ -94,9 +111,15 @@ struct Tile {
   using TF = typename PH::TF;
   using TP = typename PH::TP;
   static constexpr int NB = SH::NB, NF = SH::NF, NV = PH::NV;
+  static constexpr bool PAIRS = has_pairs<PH>::value;
+  static constexpr int NP = SH::NP;
   static constexpr size_t REC_BYTES = PH::rec_bytes(NB);
+  // pair fluxes follow the records (TF[NV][NP]); the face fluxes overwrite both
+  static constexpr size_t PAIR_OFF = (REC_BYTES + 15) / 16 * 16;
+  static constexpr size_t PAIR_BYTES = PAIRS ? (size_t)NV * NP * sizeof(TF) : 0;
+  static constexpr size_t IN_BYTES = PAIRS ? PAIR_OFF + PAIR_BYTES : REC_BYTES;
   static constexpr size_t FLUX_BYTES = (size_t)NV * NF * sizeof(TF);
-  static constexpr size_t LDS_BYTES = REC_BYTES > FLUX_BYTES ? REC_BYTES : FLUX_BYTES;
+  static constexpr size_t LDS_BYTES = IN_BYTES > FLUX_BYTES ? IN_BYTES : FLUX_BYTES;
   static constexpr int ROUNDS = SH::ROUNDS;
 
   // 1. cell records of the tile and its cross-shaped halo
@@ -117,6 +140,30 @@ struct Tile {
     }
   }
 
+  // 1b. two-point fluxes of the distance-2 pairs (PH::PAIRS only): x pairs
+  // row by row, then y pairs column by column, one pair per lane per round
+  __device__ static void pairs(const Phys<TP>& P, void* lds, int tid) {
+    if constexpr (PAIRS) {
+      TF* out = (TF*)((unsigned char*)lds + PAIR_OFF);
+      for (int q = tid; q < NP; q += 256) {
+        int ca, stride, ax;
+        if (q < SH::NPX) {
+          const int fy = q / (TX + 2), px = q - fy * (TX + 2);
+          ca = (DIMS >= 2 ? (fy + G) * SH::BX : 0) + px;
+          stride = 1;
+          ax = 0;
+        } else {
+          const int r = q - SH::NPX;
+          const int fx = r / (TY + 2), py = r - fx * (TY + 2);
+          ca = py * SH::BX + fx + G;
+          stride = SH::BX;
+          ax = 1;
+        }
+        PH::pair(P, lds, NB, ca, ca + 2 * stride, ax, out, NP, q);
+      }
+    }
+  }
+
   // 2. face fluxes, one face per lane per round, kept in registers
   __device__ static void faces(const Phys<TP>& P, const void* lds, int tid, TF Fr[ROUNDS][NV],
                                unsigned& nfb) {
@@ -125,21 +172,27 @@ struct Tile {
       int f = rd * 256 + tid;
       if (rd * 256 + (tid & ~63) >= NF) break;  // whole wave past the end
       f = f < NF ? f : NF - 1;
-      int base, stride, ax;
+      int base, stride, ax, pq;
       if (f < SH::NFX) {
         const int fy = f / (TX + 1), fx = f - fy * (TX + 1);
         base = (DIMS >= 2 ? (fy + G) * SH::BX : 0) + fx;
         stride = 1;
         ax = 0;
+        pq = fy * (TX + 2) + fx;
       } else {
         const int q = f - SH::NFX;
         const int fy = q / TX, fx = q - fy * TX;
         base = fy * SH::BX + fx + G;
         stride = SH::BX;
         ax = 1;
+        pq = SH::NPX + fx * (TY + 2) + fy;
       }
       unsigned c = 0;
-      PH::face(P, lds, NB, base, stride, ax, Fr[rd], c);
+      if constexpr (PAIRS)
+        PH::face(P, lds, NB, base, stride, ax, Fr[rd], c,
+                 (const TF*)((const unsigned char*)lds + PAIR_OFF), NP, pq);
+      else
+        PH::face(P, lds, NB, base, stride, ax, Fr[rd], c);
       if (rd * 256 + tid < NF) nfb += c;
     }
   }
@@ -207,6 +260,10 @@ void stage_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDev
   const int tid = threadIdx.x;
   TL::load(A, P, g, lds, tid, blockIdx.x, blockIdx.y);
   __syncthreads();
+  if constexpr (TL::PAIRS) {
+    TL::pairs(P, lds, tid);
+    __syncthreads();
+  }
   typename PH::TF Fr[TL::ROUNDS][TL::NV];
   unsigned nfb = 0;
   TL::faces(P, lds, tid, Fr, nfb);

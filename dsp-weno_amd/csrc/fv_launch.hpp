@@ -371,6 +371,13 @@ struct StateOps {
 #define TECNO_TILE2_X 32
 #define TECNO_TILE2_Y 15
 #define TECNO_TILE1_X 255
+// tile of the mixed-precision 2D Euler stage (records + pair fluxes in LDS)
+#ifndef TECNO_TILE2M_X
+#define TECNO_TILE2M_X 16
+#endif
+#ifndef TECNO_TILE2M_Y
+#define TECNO_TILE2M_Y 15
+#endif
 
 template <class PH, int DIMS, int TX, int TY, int G>
 static hipError_t launch_stage_t(const LaunchInfo& L, const Phys<typename PH::TP>& P,
@@ -435,14 +442,15 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
     // mixed precision: kepes_tecno4_roe with sp_weno / sp_wenoc or sp_weno_dl
     if (L.fast_flux != FV_KEPES4_ROE || L.G != 2 || L.fast_rec == RV_GENERIC)
       return hipErrorNotSupported;
+    constexpr int XM = TECNO_TILE2M_X, YM = TECNO_TILE2M_Y;
     if (L.fast_rec == RV_SPWENO)
-      return launch_stage_t<Euler2Mixed<RV_SPWENO, MM_SIMT>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b,
+      return launch_stage_t<Euler2Mixed<RV_SPWENO, MM_SIMT>, 2, XM, YM, 2>(L, P, U, U0, Uo, a, b,
                                                                           last);
     if (L.mlp_mode == MM_BF16)
-      return launch_stage_t<Euler2Mixed<RV_DSP, MM_BF16>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
+      return launch_stage_t<Euler2Mixed<RV_DSP, MM_BF16>, 2, XM, YM, 2>(L, P, U, U0, Uo, a, b, last);
     if (L.mlp_mode == MM_FP16)
-      return launch_stage_t<Euler2Mixed<RV_DSP, MM_FP16>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
-    return launch_stage_t<Euler2Mixed<RV_DSP, MM_SIMT>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
+      return launch_stage_t<Euler2Mixed<RV_DSP, MM_FP16>, 2, XM, YM, 2>(L, P, U, U0, Uo, a, b, last);
+    return launch_stage_t<Euler2Mixed<RV_DSP, MM_SIMT>, 2, XM, YM, 2>(L, P, U, U0, Uo, a, b, last);
   } else {
     if (L.G == 3)
       return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 3>(L, P, U, U0, Uo,

@@ -208,6 +208,19 @@ __device__ __forceinline__ float head(const float lg[5], const float xv[5], cons
   return 0.5f * ((0.25f - 2.0f * C1) * (d1 - d0) + (0.25f - 2.0f * C2) * (d1 - d2));
 }
 
+// Sign property up to rounding.  The softmax weights put (C1, C2) in the
+// convex hull of vertices that lie in, or on the boundary of, the region
+// where the reconstructed jump has the sign of d1 (neural_network.cc:135-260),
+// so in exact arithmetic j * d1 >= 0; on a boundary vertex the exact jump is
+// 0.  A wrong-signed j no larger than the rounding bound of its two terms
+// (1e-5 of |d1 - d0| + |d1 - d2|, about 80 FP32 ulp) is that zero rounded and
+// becomes 0; a larger one is a genuine failure and is left for the SP-WENO
+// fallback.
+__device__ __forceinline__ float sign_round(float j, float d0, float d1, float d2) {
+  const float tol = 1.0e-5f * (fabsf(d1 - d0) + fabsf(d1 - d2));
+  return (j * d1 < 0.0f && fabsf(j) <= tol) ? 0.0f : j;
+}
+
 // softmax + vertex combination -> (C1, C2)
 __device__ __forceinline__ void head_c(const float lg[5], const float xv[5], const float yv[5],
                                        float& C1, float& C2) {
@@ -278,17 +291,21 @@ __device__ __forceinline__ double rcp64(double x) {
 }
 #endif
 
-// logarithmic mean with the logarithms of the arguments known per cell:
-// f = (b-a)/(b+a) = (xi-1)/(xi+1), ln xi = ln b - ln a (misc_func.h:14-32)
+// logarithmic mean with the logarithms of the arguments known per cell
+// (misc_func.h:14-32): with f = (b-a)/(b+a) and u = f^2, the reference takes
+// (a+b) / (2 F(u)), F = 1 + u/3 + u^2/5 + u^3/7, for u < 1e-2 and otherwise
+// (a+b) f / ln(b/a), which equals (b-a) / (ln b - ln a).  The branch test
+// u < 1e-2 is made as (b-a)^2 < 1e-2 (b+a)^2, so each branch needs one
+// reciprocal of its own denominator; both are evaluated and one is selected
+// (no divergent branch in a wave).
 __device__ __forceinline__ double logavg_ln(double a, double b, double la, double lb) {
-  const double f = (b - a) * rcp64(b + a);
+  const double dab = b - a, sab = b + a;
+  const bool series = dab * dab < 1.0e-2 * (sab * sab);
+  const double r = rcp64(series ? sab : lb - la);
+  const double f = dab * r;
   const double u = f * f;
-  double F;
-  if (u < 1.0e-2)
-    F = 1.0 + u * (1.0 / 3.0) + (u * u) * (1.0 / 5.0) + (u * u * u) * (1.0 / 7.0);
-  else
-    F = (lb - la) * rcp64(2.0 * f);
-  return 0.5 * (a + b) * rcp64(F);
+  const double F = 1.0 + u * (1.0 / 3.0) + (u * u) * (1.0 / 5.0) + (u * u * u) * (1.0 / 7.0);
+  return series ? 0.5 * sab * rcp64(F) : dab * r;
 }
 
 // ---------------------------------------------------------------- the face
@@ -301,10 +318,17 @@ enum { R_RHO = 0, R_UX, R_UY, R_BETA, R_LRHO, R_LBETA, R_VF, R_N };
 template <int RV, int MM>
 struct Euler2Mixed {
   using TS = double;
-  // waves per SIMD the register budget is cut to: two for the network paths
-  // (latency hiding of the matrix-core and transcendental chains without
-  // spilling), the compiler's choice otherwise
-  static constexpr int MIN_WAVES = RV == RV_DSP ? 2 : 1;
+  // waves per SIMD the register budget is cut to: four for the network paths
+  // (the 16x15 tile needs 38 KB of LDS, so four workgroups fit a CU; the
+  // matrix-core and transcendental chains need the extra waves for latency
+  // hiding, which outweighs a few spilled registers -- measured on the KH
+  // 2048^2 stage: 4 waves 0.448 ms, 3 waves 0.488 ms), the compiler's choice
+  // otherwise
+#ifdef TECNO_MIXED_MIN_WAVES
+  static constexpr int MIN_WAVES = TECNO_MIXED_MIN_WAVES;
+#else
+  static constexpr int MIN_WAVES = RV == RV_DSP ? 4 : 1;
+#endif
   using TF = double;
   using TP = float;
   static constexpr int G = 2;
@@ -355,8 +379,48 @@ struct Euler2Mixed {
     }
   }
 
+  // two-point flux of the distance-2 pair (ca, cb) in the frame of axis ax
+  __device__ static void pair_flux(const Phys<float>& P, const double* ld, int NB, int ca, int cb,
+                                   int ax, double f[4]) {
+    const int rn = ax ? R_UY : R_UX, rt = ax ? R_UX : R_UY;
+    kepec(P, ld[R_RHO * NB + ca], ld[rn * NB + ca], ld[rt * NB + ca], ld[R_BETA * NB + ca],
+          ld[R_LRHO * NB + ca], ld[R_LBETA * NB + ca], ld[R_RHO * NB + cb], ld[rn * NB + cb],
+          ld[rt * NB + cb], ld[R_BETA * NB + cb], ld[R_LRHO * NB + cb], ld[R_LBETA * NB + cb], f);
+  }
+
+  // the pair fluxes of a tile, double [4][NP] (one per pair, shared by the
+  // two faces whose stencils hold the pair)
+  static constexpr bool PAIRS = true;
+  __device__ static void pair(const Phys<float>& P, const void* ldsv, int NB, int ca, int cb,
+                              int ax, double* out, int NP, int q) {
+    double f[4];
+    pair_flux(P, (const double*)ldsv, NB, ca, cb, ax, f);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v * NP + q] = f[v];
+  }
+
+  // a face whose pair fluxes are in LDS: pairs q (cells 0,2) and q+1 (1,3)
+  __device__ static void face(const Phys<float>& P, const void* ldsv, int NB, int base, int stride,
+                              int ax, double F[4], unsigned& nfb, const double* pairs, int NP,
+                              int q) {
+    face_core(P, ldsv, NB, base, stride, ax, F, nfb, [&](int pr, double f0[4]) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q + pr];
+    });
+  }
+
+  // a lone face (batched seam): the pair fluxes are evaluated here
   __device__ static void face(const Phys<float>& P, const void* ldsv, int NB, int base, int stride,
                               int ax, double F[4], unsigned& nfb) {
+    face_core(P, ldsv, NB, base, stride, ax, F, nfb, [&](int pr, double f0[4]) {
+      pair_flux(P, (const double*)ldsv, NB, base + pr * stride, base + (pr + 2) * stride, ax, f0);
+    });
+  }
+
+  template <class PairSrc>
+  __device__ static void face_core(const Phys<float>& P, const void* ldsv, int NB, int base,
+                                   int stride, int ax, double F[4], unsigned& nfb,
+                                   PairSrc&& pair_src) {
     const double* ld = (const double*)ldsv;
     const int rn = ax ? R_UY : R_UX, rt = ax ? R_UX : R_UY;
     // Order of work keeps the register footprint small: the eigensystem and
@@ -451,11 +515,15 @@ struct Euler2Mixed {
         const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
         float st[5], xv[5], yv[5], lg[5];
         fast::features(d0, d1, d2, maxabs, st, xv, yv);
+        // a skipped stencil (zero centre jump) feeds zeros: its ratios are not
+        // finite, and the partner lane of its matrix column would see them.
+        // Otherwise the features are finite for a finite state (tanh of an
+        // overflowing ratio saturates to +-1).
         const bool skip = !(fabsf(d1) >= 1e-15f);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) st[k] = (skip || !(fabsf(st[k]) < INFINITY)) ? 0.0f : st[k];
+        for (int k = 0; k < 5; ++k) st[k] = skip ? 0.0f : st[k];
         net.eval(st, lg);
-        float j = fast::head(lg, xv, yv, d0, d1, d2);
+        float j = fast::sign_round(fast::head(lg, xv, yv, d0, d1, d2), d0, d1, d2);
         if (skip) j = 0.0f;
         bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
 #pragma unroll
@@ -476,7 +544,7 @@ struct Euler2Mixed {
         float st[5], xv[5], yv[5];
         fast::features(d0, d1, d2, maxabs, st, xv, yv);
         fast::body5(P.net, st);
-        float j = fast::head(st, xv, yv, d0, d1, d2);
+        float j = fast::sign_round(fast::head(st, xv, yv, d0, d1, d2), d0, d1, d2);
         if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
         bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
 #pragma unroll
@@ -495,7 +563,7 @@ struct Euler2Mixed {
           float st[5], xv[5], yv[5];
           fast::features(d0, d1, d2, maxabs, st, xv, yv);
           fast::body5(P.net, st);
-          float j = fast::head(st, xv, yv, d0, d1, d2);
+          float j = fast::sign_round(fast::head(st, xv, yv, d0, d1, d2), d0, d1, d2);
           if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
           bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
           jmp[i] = j;
@@ -534,10 +602,7 @@ struct Euler2Mixed {
     for (int v = 0; v < 4; ++v) Fe[v] = (4.0 / 3.0) * Fc[v];
 #pragma unroll
     for (int pr = 0; pr < 2; ++pr) {
-      const int ca = base + pr * stride, cb = base + (pr + 2) * stride;  // (0,2), (1,3)
-      kepec(P, ld[R_RHO * NB + ca], ld[rn * NB + ca], ld[rt * NB + ca], ld[R_BETA * NB + ca],
-            ld[R_LRHO * NB + ca], ld[R_LBETA * NB + ca], ld[R_RHO * NB + cb], ld[rn * NB + cb],
-            ld[rt * NB + cb], ld[R_BETA * NB + cb], ld[R_LRHO * NB + cb], ld[R_LBETA * NB + cb], f0);
+      pair_src(pr, f0);  // F(0,2), then F(1,3)
 #pragma unroll
       for (int v = 0; v < 4; ++v) Fe[v] -= (1.0 / 6.0) * f0[v];
     }

@@ -51,6 +51,28 @@ struct Net5 {
     else
       return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, x, c, 0, 0, 0);
   }
+  // two floats -> one dword of two 16-bit values (one packed conversion)
+  __device__ static unsigned cvt2(float a, float b) {
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef E e16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, e16x2));
+  }
+  // ReLU after the conversion: a bf16 / fp16 value is negative exactly when
+  // its bit pattern is a negative 16-bit integer, so max(bits, 0) as packed
+  // int16 (two per instruction) clears the negative activations, -0 included
+  __device__ static unsigned relu2(unsigned w) {
+    typedef short s16x2 __attribute__((ext_vector_type(2)));
+    const s16x2 b = __builtin_elementwise_max(__builtin_bit_cast(s16x2, w), (s16x2){0, 0});
+    return __builtin_bit_cast(unsigned, b);
+  }
+  // next layer's input: ReLU of five outputs, the constant-1 bias input at 5
+  __device__ static V pack_relu(float x0, float x1, float x2, float x3, float x4) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 w = {relu2(cvt2(x0, x1)), relu2(cvt2(x2, x3)), relu2(cvt2(x4, 0.0f)) |
+                                                                   (cvt2(1.0f, 0.0f) << 16),
+                     0u};
+    return __builtin_bit_cast(V, w);
+  }
   // all 64 lanes of the wave must call this together
   __device__ void eval(const float in[5], float out[5]) const {
     V x = pack(in[0], in[1], in[2], in[3], in[4]);
@@ -59,8 +81,7 @@ struct Net5 {
     for (int l = 0; l < 4; ++l) {
       const f32x16 d = mma(A[l], x, zero);
       if (l < 3) {
-        x = pack(fmaxf(d[0], 0.f), fmaxf(d[1], 0.f), fmaxf(d[2], 0.f), fmaxf(d[3], 0.f),
-                 fmaxf(d[4], 0.f));
+        x = pack_relu(d[0], d[1], d[2], d[3], d[4]);
       } else {
 #pragma unroll
         for (int j = 0; j < 5; ++j) out[j] = d[j];

@@ -336,6 +336,52 @@ def ensemble_bench(tecno, use_graph=True):
                                                            g["tuned_ms"].items()}) for g in groups])
 
 
+def vortex_state(n, t=0.0, strength=5.0, gamma=1.4):
+    """Isentropic vortex on the periodic [-5,5]^2 (IsentropicVortex/param.in:
+    strength 5, free stream (1, 0), rho = p = 1), translated to time t."""
+    h = 10.0 / n
+    x = -5 + (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(x, x)
+    Xs = (X - t + 5) % 10 - 5
+    r2 = Xs ** 2 + Y ** 2
+    T = 1.0 - (gamma - 1) * strength ** 2 / (8 * gamma * np.pi ** 2) * np.exp(1 - r2)
+    e = np.exp(0.5 * (1 - r2))
+    return np.stack([T ** (1 / (gamma - 1)), 1.0 - strength / (2 * np.pi) * Y * e,
+                     strength / (2 * np.pi) * Xs * e, T ** (gamma / (gamma - 1))]).reshape(4, -1)
+
+
+def l1_gate(tecno, ns=(128, 256, 512), t_end=1.0):
+    """Equal-error gate on the isentropic vortex (exact solution): L1(rho) at
+    t = 1 on each mesh and the observed orders, for the headline path (mixed,
+    DSP-WENO, bf16 network), the same method in FP64 (reference arithmetic)
+    and the FP64 SP-WENO path."""
+    paths = {"headline": (tecno.FP32, "sp_weno_dl", tecno.MLP_BF16),
+             "fp64_dsp": (tecno.FP64, "sp_weno_dl", tecno.MLP_FP64),
+             "fp64_spweno": (tecno.FP64, "sp_weno", tecno.MLP_FP64)}
+    out = {"case": "isentropic vortex, strength 5, periodic [-5,5]^2, t=1, L1(rho)", "n": list(ns)}
+    for name, (prec, rec, mlp) in paths.items():
+        errs = []
+        for n in ns:
+            cfg = tecno.make_cfg(n=(n, n, 2), dims=2, lo=(-5, -5, 0), hi=(5, 5, 1), recon=rec,
+                                 flux="kepes_tecno4_roe", bc="periodic", cfl=0.5, precision=prec,
+                                 mlp_dtype=mlp, sign_fallback=int(prec == tecno.FP32))
+            s = tecno.Solver(cfg)
+            if rec == "sp_weno_dl":
+                s.load_net(NET_FILE)
+            s.set_state(vortex_state(n))
+            s.advance(t_end, 1)
+            t, _ = s.time()
+            errs.append(float(s.l1_error(vortex_state(n, t))[0]))
+            s.close()
+        out[name] = {"l1": errs,
+                     "order": [round(float(np.log(a / b) / np.log(2)), 3)
+                               for a, b in zip(errs[:-1], errs[1:])]}
+    h, d, b = out["headline"]["l1"], out["fp64_dsp"]["l1"], out["fp64_spweno"]["l1"]
+    out["l1_ratio_vs_fp64_dsp"] = [round(x / y, 4) for x, y in zip(h, d)]
+    out["l1_ratio_vs_fp64_spweno"] = [round(x / y, 4) for x, y in zip(h, b)]
+    return out
+
+
 def load_profile(kind, variant):
     p = os.path.join(ROOT, "profiles", f"{kind}_{variant}.json")
     if os.path.exists(p):
@@ -477,6 +523,8 @@ def main():
         out["valu_roofline"] = {"achieved": rate, "peak": VALU_PEAK,
                                 "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK,
                                 "insts_per_launch": valu["valu_insts_per_dispatch"]}
+    out["l1_gate"] = l1_gate(tecno)
+    print(f"# equal-error gate: {out['l1_gate']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -372,6 +372,14 @@ struct StateOps {
 #define TECNO_TILE2_Y 15
 #define TECNO_TILE1_X 255
 // tile of the mixed-precision 2D Euler stage (records + pair fluxes in LDS)
+// tile of the FP64 2D Euler stage with kepes_tecno4_roe (records + pair
+// fluxes in LDS: 44.7 KB at 16x15, three workgroups per CU)
+#ifndef TECNO_TILE2D_X
+#define TECNO_TILE2D_X 16
+#endif
+#ifndef TECNO_TILE2D_Y
+#define TECNO_TILE2D_Y 15
+#endif
 #ifndef TECNO_TILE2M_X
 #define TECNO_TILE2M_X 16
 #endif
@@ -455,11 +463,12 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
     if (L.G == 3)
       return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 3>(L, P, U, U0, Uo,
                                                                                  a, b, last);
+    constexpr int XD = TECNO_TILE2D_X, YD = TECNO_TILE2D_Y;
     if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_SPWENO)
-      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_SPWENO>, 2, X2, Y2, 2>(L, P, U, U0,
+      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_SPWENO>, 2, XD, YD, 2>(L, P, U, U0,
                                                                                     Uo, a, b, last);
     if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_DSP)
-      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_DSP>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
+      return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_DSP>, 2, XD, YD, 2>(L, P, U, U0, Uo,
                                                                                  a, b, last);
     return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
                                                                                a, b, last);

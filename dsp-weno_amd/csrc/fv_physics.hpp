@@ -175,11 +175,33 @@ enum { RV_SPWENO = 0, RV_DSP = 1, RV_GENERIC = 2 };
 template <typename T, int G, int FV, int RV>
 struct Euler2Ph {
   using TS = T;
-  static constexpr int MIN_WAVES = 1;
+  // waves per SIMD the register budget is cut to: three for DSP-WENO (its
+  // network chain needs the latency hiding; KH 2048^2 stage 1.29 ms vs 1.43
+  // ms at the compiler's choice), the compiler's choice otherwise
+#ifdef TECNO_EULER2_MIN_WAVES
+  static constexpr int MIN_WAVES = TECNO_EULER2_MIN_WAVES;
+#else
+  static constexpr int MIN_WAVES = (RV == RV_DSP && FV == FV_KEPES4_ROE) ? 3 : 1;
+#endif
   using TF = T;
   using TP = T;
   static constexpr int NV = 4, NREC = E_NREC;
   static constexpr size_t rec_bytes(int NB) { return (size_t)NREC * NB * sizeof(T); }
+  // kepes_tecno4_roe: the distance-2 KEPEC fluxes F(0,2), F(1,3) of a face's
+  // stencil are evaluated once per cell pair (with the face area, exactly as
+  // the face would) and shared by the two faces that use the pair
+  static constexpr bool PAIRS = FV == FV_KEPES4_ROE;
+  __device__ static void pair(const Phys<T>& P, const void* ldsv, int NB, int ca, int cb, int ax,
+                              T* out, int NP, int q) {
+    const T* lds = (const T*)ldsv;
+    const int rn = ax ? E_UY : E_UX, rt = ax ? E_UX : E_UY;
+    T f0[4];
+    kepec2(P, lds[E_RHO * NB + ca], lds[rn * NB + ca], lds[rt * NB + ca], lds[E_BK * NB + ca],
+           lds[E_RHO * NB + cb], lds[rn * NB + cb], lds[rt * NB + cb], lds[E_BK * NB + cb],
+           P.area[ax], f0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[v * NP + q] = f0[v];
+  }
   __device__ static void load(const Phys<T>& P, const T* U, int64_t N, int64_t idx, int flipx,
                               int flipy, void* lds, int NB, int c) {
     T rec[NREC];
@@ -193,8 +215,11 @@ struct Euler2Ph {
   // scaled variable and eigenvector entries both change sign on y faces, so
   // the dissipation is unchanged bit for bit.  Sums over the x and y
   // components keep the reference's x-before-y order via `ax`.
+  // pairs: the shared pair fluxes (pairs q and q+1 are this face's F(0,2)
+  // and F(1,3)); nullptr evaluates them here
   __device__ static void face(const Phys<T>& P, const void* ldsv, int NB, int base, int stride,
-                              int ax, T F[4], unsigned& nfb) {
+                              int ax, T F[4], unsigned& nfb, const T* pairs = nullptr,
+                              int NP = 0, int q = 0) {
     const T* lds = (const T*)ldsv;
     constexpr int S = 2 * G;
     const int rn = ax ? E_UY : E_UX, rt = ax ? E_UX : E_UY;
@@ -219,10 +244,22 @@ struct Euler2Ph {
       kepec2(P, rho[L], un[L], ut[L], bk[L], rho[R], un[R], ut[R], bk[R], area, f0, &rhoh, &betah);
 #pragma unroll
       for (int v = 0; v < 4; ++v) Fe[v] = T(0) + f0[v] * T(4.0 / 3.0);
-      kepec2(P, rho[L - 1], un[L - 1], ut[L - 1], bk[L - 1], rho[R], un[R], ut[R], bk[R], area, f0);
+      if (G == 2 && pairs) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q];
+      } else {
+        kepec2(P, rho[L - 1], un[L - 1], ut[L - 1], bk[L - 1], rho[R], un[R], ut[R], bk[R], area,
+               f0);
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v) Fe[v] += f0[v] * T(-1.0 / 6.0);
-      kepec2(P, rho[L], un[L], ut[L], bk[L], rho[R + 1], un[R + 1], ut[R + 1], bk[R + 1], area, f0);
+      if (G == 2 && pairs) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q + 1];
+      } else {
+        kepec2(P, rho[L], un[L], ut[L], bk[L], rho[R + 1], un[R + 1], ut[R + 1], bk[R + 1], area,
+               f0);
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v) Fe[v] += f0[v] * T(-1.0 / 6.0);
     } else {

@@ -184,6 +184,111 @@ __device__ __forceinline__ void features(float d0, float d1, float d2, float max
   vertices(tm, tp, fmaxf(fmaxf(a1, a2), a3), xv, yv);
 }
 
+// features without the vertex lists: the ratios and the largest scaled jump
+// are returned for vertex_combine
+__device__ __forceinline__ void features_nv(float d0, float d1, float d2, float maxabs,
+                                            float st[5], float& tm, float& tp, float& maxd) {
+  const float inv = rcp(fmaxf(1.0f, maxabs));
+  tm = quot(d2, d1);
+  tp = quot(d0, d1);
+  const float a1 = fabsf(d0) * inv, a2 = fabsf(d1) * inv, a3 = fabsf(d2) * inv;
+  st[0] = tanh(tm);
+  st[1] = tanh(tp);
+  st[2] = a1;
+  st[3] = a2;
+  st[4] = a3;
+  maxd = fmaxf(fmaxf(a1, a2), a3);
+}
+
+// (C1, C2) = sum_i s_i (xv_i, yv_i) / sum_i s_i for the vertex lists of
+// compute_vertices (neural_network.cc:135-260, vertex_mode 2, permutation
+// 2; the same region tree as `vertices`), with the weighted sum taken inside
+// each region: every list uses at most three distinct values per
+// coordinate, so a region costs a few products instead of ten vertex
+// assignments, and no vertex list is live across the network.
+__device__ __forceinline__ void vertex_combine(float tm, float tp, float maxd, const float s[5],
+                                               float& C1, float& C2) {
+  const float thr = 1e-8f;
+  const float S = (s[0] + s[1]) + (s[2] + s[3]) + s[4];
+  const float e = 0.125f, t = -0.375f, m = -0.125f;
+  float c1, c2;
+  if ((tm - 1.0f) > thr && (tp - 1.0f) > thr) {
+    c1 = e * S;
+    c2 = e * S;
+  } else if (((tm - 1.0f) > thr && (tp - 1.0f) < -thr) ||
+             ((tm - 1.0f) < -thr && (tp - 1.0f) > thr)) {
+    const bool r2 = (tm - 1.0f) > thr;  // theta- > 1, theta+ < 1 (else the mirror case)
+    const float psi_p = quot(1.0f - tm, 1.0f - tp);
+    const float psi_m = quot(1.0f, psi_p);
+    const float g1 = fminf(maxd, 0.125f);
+    const float g2 = -fminf(maxd, 0.375f);
+    const float x1 = 0.125f * (1.0f + psi_p) - g1 * psi_p;
+    const float y2 = 0.125f * (1.0f + psi_m) - g1 * psi_m;
+    if (psi_p < -1.0f) {
+      if (r2 && x1 < g2) {  // all vertices (nx, ny)
+        const float nx = ((psi_m * psi_m) + psi_m) * rcp(8.0f * ((psi_m * psi_m) + 1.0f));
+        c1 = nx * S;
+        c2 = (0.125f * (1.0f + psi_m) - nx * psi_m) * S;
+      } else if (!r2 && x1 < g2) {  // (g2,g1,g2,g1,0), (g1,g1,g2,g2,0)
+        c1 = g2 * (s[0] + s[2]) + g1 * (s[1] + s[3]);
+        c2 = g1 * (s[0] + s[1]) + g2 * (s[2] + s[3]);
+      } else if (r2) {  // (g2,x1,g2,cx,cx), (g1,g1,y2,cy,cy)
+        const float cx = (2.0f * g2 + x1) * (1.0f / 3.0f), cy = (2.0f * g1 + y2) * (1.0f / 3.0f);
+        c1 = g2 * (s[0] + s[2]) + x1 * s[1] + cx * (s[3] + s[4]);
+        c2 = g1 * (s[0] + s[1]) + y2 * s[2] + cy * (s[3] + s[4]);
+      } else {  // (g1,g1,g2,x1,g2), (g2,g1,g2,g1,y1)
+        const float y1 = 0.125f * (1.0f + psi_m) - g2 * psi_m;
+        c1 = g1 * (s[0] + s[1]) + g2 * (s[2] + s[4]) + x1 * s[3];
+        c2 = g2 * (s[0] + s[2]) + g1 * (s[1] + s[3]) + y1 * s[4];
+      }
+    } else {
+      if (r2 && y2 < g2) {  // (g2,g1,g2,g1,0), (g1,g1,g2,g2,0)
+        c1 = g2 * (s[0] + s[2]) + g1 * (s[1] + s[3]);
+        c2 = g1 * (s[0] + s[1]) + g2 * (s[2] + s[3]);
+      } else if (!r2 && y2 < g2) {  // all vertices (nx, ny)
+        const float nx = ((psi_m * psi_m) + psi_m) * rcp(8.0f * ((psi_m * psi_m) + 1.0f));
+        c1 = nx * S;
+        c2 = (0.125f * (1.0f + psi_m) - nx * psi_m) * S;
+      } else if (r2) {  // (g2,g1,g2,g1,x2), (g1,g1,g2,y2,g2)
+        const float x2 = 0.125f * (1.0f + psi_p) - g2 * psi_p;
+        c1 = g2 * (s[0] + s[2]) + g1 * (s[1] + s[3]) + x2 * s[4];
+        c2 = g1 * (s[0] + s[1]) + g2 * (s[2] + s[4]) + y2 * s[3];
+      } else {  // (g1,x1,g1,cx,cx), (g2,g2,y2,cy,cy)
+        const float cx = (2.0f * g1 + x1) * (1.0f / 3.0f), cy = (2.0f * g2 + y2) * (1.0f / 3.0f);
+        c1 = g1 * (s[0] + s[2]) + x1 * s[1] + cx * (s[3] + s[4]);
+        c2 = g2 * (s[0] + s[1]) + y2 * s[2] + cy * (s[3] + s[4]);
+      }
+    }
+  } else if (fabsf(tm - 1.0f) < thr && (tp - 1.0f) > thr) {  // (e,e,e,e,e), (t,e,e,t,m)
+    c1 = e * S;
+    c2 = t * (s[0] + s[3]) + e * (s[1] + s[2]) + m * s[4];
+  } else if (fabsf(tp - 1.0f) < thr && (tm - 1.0f) > thr) {  // (e,e,t,t,m), (e,e,e,e,e)
+    c1 = e * (s[0] + s[1]) + t * (s[2] + s[3]) + m * s[4];
+    c2 = e * S;
+  } else {  // (e,e,t,t,m), (e,t,t,e,m)
+    c1 = e * (s[0] + s[1]) + t * (s[2] + s[3]) + m * s[4];
+    c2 = e * (s[0] + s[3]) + t * (s[1] + s[2]) + m * s[4];
+  }
+  const float inv = rcp(S);
+  C1 = c1 * inv;
+  C2 = c2 * inv;
+}
+
+// softmax (zero-initialised maximum, neural_network.cc:276-295) of the
+// logits, then the vertex combination -> reconstructed jump
+__device__ __forceinline__ float head_combine(const float lg[5], float tm, float tp, float maxd,
+                                              float d0, float d1, float d2) {
+  float mx = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) mx = fmaxf(mx, lg[i]);
+  float s[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) s[i] = exp(lg[i] - mx);
+  float C1, C2;
+  vertex_combine(tm, tp, maxd, s, C1, C2);
+  return 0.5f * ((0.25f - 2.0f * C1) * (d1 - d0) + (0.25f - 2.0f * C2) * (d1 - d2));
+}
+
 // softmax with zero-initialised maximum + vertex weights -> reconstructed jump
 __device__ __forceinline__ float head(const float lg[5], const float xv[5], const float yv[5],
                                       float d0, float d1, float d2) {
@@ -445,11 +550,15 @@ struct Euler2Mixed {
     float vnf, af, s0, s1, Rn0f, Rt0f, RE0f, Rn1f, Rt1f, RE1f, Rt2f, RE2f, Rn4f, RE4f;
     {
       const double rhoh = aux[0], betah = aux[1], ph = aux[2], vn = aux[3], vt = aux[4];
+      // c1 and S2 in FP32 (one instruction each instead of a scaled FP64
+      // sequence): they scale whole rows of R~^T, so their rounding scales a
+      // projected jump without disturbing the cancellation inside it.  The
+      // sound speed a sits inside the acoustic rows and stays FP64.
       const double a = sqrt(0.5 * P.gamma_d * rcp64(betah));
       const double vsq = vn * vn + vt * vt;
       const double H = a * a * P.inv_gm1_d + 0.5 * vsq;
-      const double c1 = sqrt(rhoh * P.inv_gamma_d);
-      const double S0 = P.s05_d * c1, S1 = P.sgm1_d * c1, S2 = sqrt(ph);
+      const double c1 = (double)fast::sqrt((float)(rhoh * P.inv_gamma_d));
+      const double S0 = P.s05_d * c1, S1 = P.sgm1_d * c1, S2 = (double)fast::sqrt((float)ph);
       const double Rn0 = (vn - a) * S0, Rt0 = vt * S0, RE0 = (H - a * vn) * S0;
       const double Rn1 = vn * S1, Rt1 = vt * S1, RE1 = (0.5 * vsq) * S1;
       const double Rt2 = -S2, RE2 = -vt * S2;
@@ -513,8 +622,8 @@ struct Euler2Mixed {
         const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
         const float z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;
         const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
-        float st[5], xv[5], yv[5], lg[5];
-        fast::features(d0, d1, d2, maxabs, st, xv, yv);
+        float st[5], lg[5], tm, tp, maxd;
+        fast::features_nv(d0, d1, d2, maxabs, st, tm, tp, maxd);
         // a skipped stencil (zero centre jump) feeds zeros: its ratios are not
         // finite, and the partner lane of its matrix column would see them.
         // Otherwise the features are finite for a finite state (tanh of an
@@ -523,7 +632,7 @@ struct Euler2Mixed {
 #pragma unroll
         for (int k = 0; k < 5; ++k) st[k] = skip ? 0.0f : st[k];
         net.eval(st, lg);
-        float j = fast::sign_round(fast::head(lg, xv, yv, d0, d1, d2), d0, d1, d2);
+        float j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, d0, d1, d2), d0, d1, d2);
         if (skip) j = 0.0f;
         bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
 #pragma unroll
@@ -541,10 +650,10 @@ struct Euler2Mixed {
         const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
         const float z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;
         const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
-        float st[5], xv[5], yv[5];
-        fast::features(d0, d1, d2, maxabs, st, xv, yv);
+        float st[5], tm, tp, maxd;
+        fast::features_nv(d0, d1, d2, maxabs, st, tm, tp, maxd);
         fast::body5(P.net, st);
-        float j = fast::sign_round(fast::head(st, xv, yv, d0, d1, d2), d0, d1, d2);
+        float j = fast::sign_round(fast::head_combine(st, tm, tp, maxd, d0, d1, d2), d0, d1, d2);
         if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
         bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
 #pragma unroll
@@ -560,10 +669,10 @@ struct Euler2Mixed {
         } else {
           const float z0 = z1 - d0, z3 = z2 + d2;
           const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
-          float st[5], xv[5], yv[5];
-          fast::features(d0, d1, d2, maxabs, st, xv, yv);
+          float st[5], tm, tp, maxd;
+          fast::features_nv(d0, d1, d2, maxabs, st, tm, tp, maxd);
           fast::body5(P.net, st);
-          float j = fast::sign_round(fast::head(st, xv, yv, d0, d1, d2), d0, d1, d2);
+          float j = fast::sign_round(fast::head_combine(st, tm, tp, maxd, d0, d1, d2), d0, d1, d2);
           if (fabsf(d1) < 1e-15f) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
           bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
           jmp[i] = j;

@@ -1,0 +1,109 @@
+"""Host build of the mixed-precision face numerics (fv_mixed.hpp compiled
+with g++ under TECNO_HOST_BUILD, no GPU): the fused vertex combination
+against the vertex lists of compute_vertices, the rounding-aware sign rule,
+the residual-corrected quotient, and the per-cell-logarithm log-mean against
+the oracle's logavg (misc_func.h:14-32)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle_ctypes as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "dsp-weno_amd", "csrc")
+
+HARNESS = r"""
+#define TECNO_HOST_BUILD 1
+#include <cstdio>
+#include <cstdlib>
+#include <cmath>
+#include <random>
+#include "fv_mixed.hpp"
+using namespace tecno;
+int main() {
+  std::mt19937 g(11);
+  std::uniform_real_distribution<float> U(-3.0f, 3.0f);
+  const float sp[] = {1.0f, 1.0f + 1e-8f, 1.0f - 1e-8f, 1.0f + 2e-8f, 0.0f, -1.0f, 2.0f,
+                      1.0000001f, 0.9999999f, 0.5f, -0.5f};
+  double dmax = 0.0;
+  for (int it = 0; it < 300000; ++it) {
+    float tm = U(g), tp = U(g), md = std::fabs(U(g)) * 0.1f;
+    if (it % 7 == 0) tm = sp[it % 11];
+    if (it % 5 == 0) tp = sp[(it / 5) % 11];
+    if (it % 13 == 0) md = 0.0f;
+    float lg[5], xv[5], yv[5], s[5], C1a, C2a, C1b, C2b;
+    for (int i = 0; i < 5; ++i) lg[i] = U(g);
+    fast::vertices(tm, tp, md, xv, yv);
+    fast::head_c(lg, xv, yv, C1a, C2a);
+    float mx = 0.0f;
+    for (int i = 0; i < 5; ++i) mx = fmaxf(mx, lg[i]);
+    for (int i = 0; i < 5; ++i) s[i] = fast::exp(lg[i] - mx);
+    fast::vertex_combine(tm, tp, md, s, C1b, C2b);
+    dmax = std::fmax(dmax, std::fmax(std::fabs(C1a - C1b), std::fabs(C2a - C2b)));
+  }
+  std::printf("combine %.6e\n", dmax);
+  // sign rule: tiny wrong-signed jump -> 0; large wrong-signed kept; right sign kept
+  std::printf("sign %g %g %g\n", fast::sign_round(-1e-9f, 1.0f, 1.5f, 1.2f),
+              fast::sign_round(-0.1f, 1.0f, 1.5f, 1.2f), fast::sign_round(0.2f, 1.0f, 1.5f, 1.2f));
+  // quotient: exact for representable quotients (equal / opposite jumps)
+  int exact = 1;
+  for (int it = 0; it < 100000; ++it) {
+    float d = U(g) * std::pow(10.0f, (float)(it % 20 - 10));
+    if (d == 0.0f) continue;
+    exact &= fast::quot(d, d) == 1.0f && fast::quot(-d, d) == -1.0f && fast::quot(2.0f * d, d) == 2.0f;
+  }
+  std::printf("quot %d\n", exact);
+  // log-mean from per-cell logarithms, for the caller to compare with the oracle
+  std::mt19937_64 h(5);
+  std::uniform_real_distribution<double> V(0.2, 5.0);
+  FILE* f = std::fopen(std::getenv("LOGAVG_OUT"), "w");
+  for (int it = 0; it < 20000; ++it) {
+    double a = V(h), b = (it % 3 == 0) ? a * (1.0 + (V(h) - 2.6) * 1e-3) : V(h);
+    std::fprintf(f, "%.17g %.17g %.17g\n", a, b, logavg_ln(a, b, std::log(a), std::log(b)));
+  }
+  std::fclose(f);
+  return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def harness_output():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "h.cpp")
+        exe = os.path.join(d, "h")
+        with open(src, "w") as f:
+            f.write(HARNESS)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-I", CSRC, src, "-o", exe], check=True)
+        la = os.path.join(d, "logavg.txt")
+        out = subprocess.run([exe], check=True, capture_output=True, text=True,
+                             env=dict(os.environ, LOGAVG_OUT=la)).stdout
+        vals = np.loadtxt(la)
+    return out, vals
+
+
+def test_vertex_combine_matches_vertex_lists(harness_output):
+    out, _ = harness_output
+    d = float(out.split("combine")[1].split()[0])
+    assert d <= 1e-6
+
+
+def test_sign_round(harness_output):
+    out, _ = harness_output
+    vals = [float(x) for x in out.split("sign")[1].split()[:3]]
+    assert vals[0] == 0.0 and vals[1] == pytest.approx(-0.1) and vals[2] == pytest.approx(0.2)
+
+
+def test_quotient_exact_for_representable(harness_output):
+    out, _ = harness_output
+    assert out.split("quot")[1].split()[0] == "1"
+
+
+def test_logavg_from_cell_logs_matches_oracle(harness_output):
+    _, vals = harness_output
+    ref = orc.logavg(vals[:, 0], vals[:, 1])
+    rel = np.abs(vals[:, 2] - ref) / np.abs(ref)
+    assert rel.max() <= 4e-15

@@ -567,9 +567,12 @@ struct Tile3 {
 };
 
 #ifndef TECNO_HOST_BUILD
+#ifndef TECNO_STAGE3_WAVES
+#define TECNO_STAGE3_WAVES 2
+#endif
 template <class PH, int TX, int TY, int TZ, int G, bool VISC>
-__global__ __launch_bounds__(256) void stage3_kernel(StageArgs<typename PH::TS> A,
-                                                     Phys<typename PH::TP> P, GridDev g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TECNO_STAGE3_WAVES)))
+void stage3_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDev g) {
   using TL = Tile3<PH, TX, TY, TZ, G, VISC>;
   using T = typename PH::TS;
   __shared__ __attribute__((aligned(16))) unsigned char lds[TL::LDS_BYTES];

@@ -21,6 +21,7 @@ softmax-vertex head, Adam with betas (0.5, 0.9), weight decay 1e-5); the
 arithmetic is the device trainer's (bf16 activations, FP32 master weights).
 
     python retrain.py --out net.txt [--stencils 1e7] [--epochs 40] [--batch 65536]
+    python retrain.py --out net.txt --data DSP-WENO_training_data.dat
 """
 import argparse
 import json
@@ -144,6 +145,48 @@ def harvest_euler_2d(prim, exact_face, drop_rel=1e-12, gamma=GAMMA):
         out_u4.append(u4[keep])
         out_t.append(np.stack([tg[keep], tg[keep]], axis=1))
     return np.concatenate(out_u4), np.concatenate(out_t)
+
+
+# ---------------------------------------------------------------- data files
+def read_trainer_data(path):
+    """Rows (n x 21: 15 features, 4 cells, 2 targets, the layout of
+    tecno_make_train_rows) from a data file of the reference trainer
+    (DSP-WENO_network_trainer.py:281-333): one stencil per line, values
+    separated by commas.  With tanh features (the trainer's default,
+    config.py:14) a line has 23 values: the 4 cells, theta-, theta+,
+    tanh(theta-), tanh(theta+), the 3 scaled jump magnitudes, 5 x-vertices,
+    5 y-vertices, the left and right targets; without, 21 values (no tanh
+    columns, the network inputs starting at theta-)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            tok = [t for t in line.replace("\n", "").split(",") if t.strip()]
+            if not tok:
+                continue
+            v = [float(t) for t in tok]
+            if len(v) == 23:
+                feat = v[6:21]
+            elif len(v) == 21:
+                feat = v[4:19]
+            else:
+                raise ValueError(f"{path}: a line has {len(v)} values (expected 21 or 23)")
+            rows.append(feat + v[0:4] + v[-2:])
+    return np.array(rows, dtype=np.float64).reshape(-1, 21)
+
+
+def write_trainer_data(path, rows):
+    """The inverse of read_trainer_data in the 23-value layout: cells,
+    theta- = D2/D1, theta+ = D0/D1 of the cells, the 15 features, targets
+    (%.17g, comma separated)."""
+    rows = np.asarray(rows, dtype=np.float64)
+    c = rows[:, 15:19]
+    d0, d1, d2 = c[:, 1] - c[:, 0], c[:, 2] - c[:, 1], c[:, 3] - c[:, 2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tm, tp = d2 / d1, d0 / d1
+    out = np.column_stack([c, tm, tp, rows[:, 0:15], rows[:, 19:21]])
+    with open(path, "w") as f:
+        for r in out:
+            f.write(",".join(f"{x:.17g}" for x in r) + "\n")
 
 
 # ---------------------------------------------------------------- runs
@@ -285,11 +328,23 @@ def main():
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--init", default=None, help="start from this net_file instead of Xavier")
     ap.add_argument("--raw", action="store_true", help="train on raw (not jump-normalised) rows")
+    ap.add_argument("--data", default=None,
+                    help="train on a data file in the reference trainer's layout (its data_file) "
+                         "instead of harvesting")
+    ap.add_argument("--write-data", default=None,
+                    help="also write the harvested stencils in the reference trainer's layout")
     a = ap.parse_args()
     t0 = time.perf_counter()
-    u4, tg, n_harvest = build_dataset(a.stencils)
-    print(f"harvested {n_harvest} stencils, {len(u4)} informative kept, "
-          f"{time.perf_counter() - t0:.1f} s", flush=True)
+    if a.data:
+        rows = read_trainer_data(a.data)
+        u4, tg = rows[:, 15:19], rows[:, 19:21]
+        print(f"read {len(u4)} stencils from {a.data}", flush=True)
+    else:
+        u4, tg, n_harvest = build_dataset(a.stencils)
+        print(f"harvested {n_harvest} stencils, {len(u4)} informative kept, "
+              f"{time.perf_counter() - t0:.1f} s", flush=True)
+        if a.write_data:
+            write_trainer_data(a.write_data, tecno.make_train_rows(u4, tg))
     s, summary = train(u4, tg, epochs=a.epochs, batch=a.batch, init_net=a.init,
                        normalize=not a.raw, log=lambda m: print(m, flush=True))
     s.save_net(a.out)

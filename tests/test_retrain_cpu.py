@@ -5,6 +5,7 @@ by interpolation of exact stencils, not by restating the formulas."""
 import math
 
 import numpy as np
+import pytest
 
 import retrain as R
 
@@ -105,3 +106,30 @@ def test_normalize_rows_is_affine_and_drops_noise():
     assert np.allclose(out[:, :15], rows[keep, :15])
     assert np.allclose(out[:, 15:21], (rows[keep, 15:21] - c[:, 1:2]) / s[:, None])
     assert np.allclose(np.abs(np.diff(out[:, 15:19], axis=1)).max(axis=1), 1.0)
+
+
+def test_trainer_data_file_round_trip(tmp_path):
+    """A data file in the reference trainer's 23-column layout is read into
+    device-trainer rows and written back value for value; the feature
+    columns of a written file are the oracle's row builder's (the same
+    features the solver computes, neural_network.cc:16-77)."""
+    import oracle_ctypes as orc
+    rng = np.random.default_rng(9)
+    u4 = rng.normal(size=(50, 4))
+    tg = rng.normal(size=(50, 2))
+    rows = orc.train_rows(u4, tg)
+    p = tmp_path / "data.dat"
+    R.write_trainer_data(str(p), rows)
+    back = R.read_trainer_data(str(p))
+    assert np.array_equal(back, rows)
+    line = open(p).readline().split(",")
+    assert len(line) == 23
+    v = [float(x) for x in line]
+    assert np.allclose(v[0:4], u4[0]) and np.allclose(v[21:23], tg[0])
+    assert v[4] == pytest.approx((u4[0, 3] - u4[0, 2]) / (u4[0, 2] - u4[0, 1]))
+    # 21-column layout (no tanh features): inputs start at theta-
+    p21 = tmp_path / "d21.dat"
+    with open(p21, "w") as f:
+        for r, c, t in zip(rows, u4, tg):
+            f.write(",".join(f"{x:.17g}" for x in [*c, *r[0:15], *t]) + "\n")
+    assert np.array_equal(R.read_trainer_data(str(p21)), rows)

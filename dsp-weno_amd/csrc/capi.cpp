@@ -71,7 +71,7 @@ struct tecno_ctx {
   unsigned short* dWmf = nullptr;  // MFMA-packed weights: [bf16 | fp16][4][64][8]
   Phys<double> P64;
   Phys<float> P32;
-  LaunchInfo L;
+  LaunchInfo L{};
   bool state_set = false;
   bool configured_final = false;
   double cfg_final = 0;
@@ -90,7 +90,7 @@ static int flux_ng(int flux) {
     case TECNO_FLUX_LIN_EC4: case TECNO_FLUX_LIN_TECNO4: case TECNO_FLUX_BUR_EC4:
     case TECNO_FLUX_BUR_TECNO4: case TECNO_FLUX_KEPES_TECNO4_ROE:
     case TECNO_FLUX_KEPES_TECNO4_RUSANOV: case TECNO_FLUX_ROE_TECNO4_ROE:
-    case TECNO_FLUX_KEPEC4: case TECNO_FLUX_ROE_EC4:
+    case TECNO_FLUX_KEPEC4: case TECNO_FLUX_ROE_EC4: case TECNO_FLUX_KEP4: case TECNO_FLUX_KEPS4:
       return 2;
     default:
       return 1;
@@ -106,6 +106,9 @@ static void fill_phys(const tecno_ctx* c, Phys<T>& P) {
   P.recon = g.recon == TECNO_REC_SP_WENO ? REC_SPWENO
             : g.recon == TECNO_REC_ENO   ? REC_ENO
             : g.recon == TECNO_REC_SP_WENO_DL ? REC_DSP
+            : g.recon == TECNO_REC_SECOND ? REC_SECOND
+            : g.recon == TECNO_REC_MINMOD ? REC_MINMOD
+            : g.recon == TECNO_REC_MINABS ? REC_MINABS
                                                : REC_FIRST;
   P.corr = g.spweno_corr;
   P.eno_k = g.eno_level;
@@ -234,10 +237,10 @@ static int upload_net(tecno_ctx* c) {
 
 static bool fp64(const tecno_ctx* c) { return c->cfg.precision == TECNO_FP64; }
 
-static hipError_t launch_stage(tecno_ctx* c, const void* U, const void* U0, void* Uo, double a,
-                               double b, int last) {
-  if (fp64(c)) return Launch<double>::stage(c->L, c->P64, U, U0, Uo, a, b, last);
-  return Launch<float>::stage(c->L, c->P32, U, U0, Uo, a, b, last);
+static hipError_t launch_stage(tecno_ctx* c, const void* U, const void* U0, void* Uo,
+                               const StageCoef& k) {
+  if (fp64(c)) return Launch<double>::stage(c->L, c->P64, U, U0, Uo, k);
+  return Launch<float>::stage(c->L, c->P32, U, U0, Uo, k);
 }
 
 static hipError_t st_prim_to_con(tecno_ctx* c, const double* prim, void* U) {
@@ -307,14 +310,16 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
     return fail(TECNO_EINVAL, "scalar: 3D grids are not supported by this build");
   if (g.system == TECNO_SYS_SCALAR && (g.flux < 0 || g.flux > 7))
     return fail(TECNO_EINVAL, "unknown scalar flux");
-  if (g.system == TECNO_SYS_EULER && !((g.flux >= 100 && g.flux <= 106) || (g.flux >= 120 && g.flux <= 124)))
+  if (g.system == TECNO_SYS_EULER && !((g.flux >= 100 && g.flux <= 106) || (g.flux >= 120 && g.flux <= 128)))
     return fail(TECNO_EINVAL, "unknown Euler flux");
+  if (g.recon < TECNO_REC_SP_WENO || g.recon > TECNO_REC_MINABS)
+    return fail(TECNO_EINVAL, "unknown reconstruction");
   if (g.system == TECNO_SYS_EULER && g.model != TECNO_MODEL_EULER && !ns)
     return fail(TECNO_EINVAL, "unknown Euler model");
   if (g.recon == TECNO_REC_ENO && g.eno_level != 2 && g.eno_level != 3)
     return fail(TECNO_EINVAL, "eno k: k must be 2 or 3");
-  if (g.time_scheme != TECNO_TS_SSPRK3 && g.time_scheme != TECNO_TS_RK1)
-    return fail(TECNO_EINVAL, "only ssprk3 and rk1 time schemes are supported");
+  if (g.time_scheme < TECNO_TS_RK1 || g.time_scheme > TECNO_TS_RK4)
+    return fail(TECNO_EINVAL, "unknown time scheme");
   if (g.precision != TECNO_FP64 && g.precision != TECNO_FP32)
     return fail(TECNO_EINVAL, "precision must be FP64 or FP32");
   for (int i = 0; i < 6; ++i)
@@ -338,7 +343,7 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   c->N = (int64_t)c->n[0] * c->n[1] * c->n[2];
   c->nvar = g.system == TECNO_SYS_SCALAR ? 1 : (g.dims == 3 ? 5 : 4);
   c->nprim = c->nvar;
-  int rec_ng = g.recon == TECNO_REC_ENO ? g.eno_level : 2;
+  int rec_ng = g.recon == TECNO_REC_ENO ? g.eno_level : g.recon == TECNO_REC_FIRST ? 1 : 2;
   c->G = std::max(flux_ng(g.flux), rec_ng);
   if (c->G < 2) c->G = 2;
   for (int d = 0; d < g.dims; ++d)
@@ -350,8 +355,9 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   c->L.state_fp64 = (fp64(c) || g.system == TECNO_SYS_EULER) ? 1 : 0;
   c->esize = c->L.state_fp64 ? 8 : 4;
   if (!fp64(c) && g.system == TECNO_SYS_EULER &&
-      (g.flux != TECNO_FLUX_KEPES_TECNO4_ROE || g.ec1_fix || g.recon == TECNO_REC_ENO ||
-       g.recon == TECNO_REC_FIRST || g.spweno_corr == 2 && g.recon != TECNO_REC_SP_WENO)) {
+      (g.flux != TECNO_FLUX_KEPES_TECNO4_ROE || g.ec1_fix ||
+       (g.recon != TECNO_REC_SP_WENO && g.recon != TECNO_REC_SP_WENO_DL) ||
+       g.spweno_corr == 2 && g.recon != TECNO_REC_SP_WENO)) {
     delete c;
     return fail(TECNO_EINVAL, "FP32 Euler path: kepes_tecno4_roe with sp_weno, sp_wenoc or "
                               "sp_weno_dl only (use FP64 for the other options)");
@@ -369,6 +375,11 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   if (hipMalloc(&c->ctl, sizeof(Ctl)) != hipSuccess) {
     tecno_destroy(c);
     return fail(TECNO_ENOMEM, "device allocation failed");
+  }
+  // classical RK4 keeps the accumulated residual (FV::residual2, fv.h:221)
+  if (g.time_scheme == TECNO_TS_RK4 && hipMalloc(&c->L.R2, bytes) != hipSuccess) {
+    tecno_destroy(c);
+    return fail(TECNO_ENOMEM, "device allocation of the RK4 accumulator failed");
   }
   Ctl h{};
   (void)hipMemcpy(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice);
@@ -398,6 +409,7 @@ void tecno_destroy(tecno_ctx* c) {
   for (int i = 0; i < 3; ++i)
     if (c->U[i]) (void)hipFree(c->U[i]);
   if (c->ctl) (void)hipFree(c->ctl);
+  if (c->L.R2) (void)hipFree(c->L.R2);
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->dW64) (void)hipFree(c->dW64);
   if (c->db64) (void)hipFree(c->db64);
@@ -599,14 +611,39 @@ static int launch_step(tecno_ctx* c) {
     return 0;
   };
   int rc;
-  if (c->cfg.time_scheme == TECNO_TS_RK1) {
-    if ((rc = timed([&] { return launch_stage(c, A, A, B, 0.0, 1.0, 1); }))) return rc;
-    HIPCHK(hipMemcpyAsync(A, B, (size_t)c->N * c->nvar * c->esize, hipMemcpyDeviceToDevice, c->stream));
-  } else {
-    // SSPRK3 (ext_constants.h:7-8): a = {0, 3/4, 1/3}, b = {1, 1/4, 2/3}
-    if ((rc = timed([&] { return launch_stage(c, A, A, B, 0.0, 1.0, 0); }))) return rc;
-    if ((rc = timed([&] { return launch_stage(c, B, A, C, 3.0 / 4.0, 1.0 / 4.0, 0); }))) return rc;
-    if ((rc = timed([&] { return launch_stage(c, C, A, A, 1.0 / 3.0, 2.0 / 3.0, 1); }))) return rc;
+  auto st = [&](const void* U, void* Uo, StageCoef k) {
+    return timed([&] { return launch_stage(c, U, A, Uo, k); });
+  };
+  switch (c->cfg.time_scheme) {
+    case TECNO_TS_RK1:
+      if ((rc = st(A, B, {0.0, 1.0, 1.0, 0.0, 1, 0}))) return rc;
+      HIPCHK(hipMemcpyAsync(A, B, (size_t)c->N * c->nvar * c->esize, hipMemcpyDeviceToDevice,
+                            c->stream));
+      break;
+    case TECNO_TS_JAMESON_RK4:
+      // every stage restarts from U^n: u = U^n - (dt/vol)*c_k*res(u_prev),
+      // c = {1/4, 1/3, 1/2, 1} (fv.cc:1176-1184, ext_constants.h:9).  The last
+      // stage writes U^n in place: each cell reads only its own U^n.
+      if ((rc = st(A, B, {1.0, 0.0, 1.0 / 4.0, 0.0, 0, 0}))) return rc;
+      if ((rc = st(B, C, {1.0, 0.0, 1.0 / 3.0, 0.0, 0, 0}))) return rc;
+      if ((rc = st(C, B, {1.0, 0.0, 1.0 / 2.0, 0.0, 0, 0}))) return rc;
+      if ((rc = st(B, A, {1.0, 0.0, 1.0, 0.0, 1, 0}))) return rc;
+      break;
+    case TECNO_TS_RK4:
+      // classical RK4 (fv.cc:1185-1197, ext_constants.h:10): stage k adds
+      // res/rk4_rks[k] to the accumulator and, before the last, moves to
+      // U^n - (dt/vol)*rk4_rks[k+1]*res; the last takes U^n - (dt/vol)*R2/6
+      if ((rc = st(A, B, {1.0, 0.0, 1.0 / 2.0, 1.0, 0, 1}))) return rc;
+      if ((rc = st(B, C, {1.0, 0.0, 1.0 / 2.0, 2.0, 0, 2}))) return rc;
+      if ((rc = st(C, B, {1.0, 0.0, 1.0, 2.0, 0, 2}))) return rc;
+      if ((rc = st(B, A, {1.0, 0.0, 1.0, 1.0, 1, 3}))) return rc;
+      break;
+    default:
+      // SSPRK3 (ext_constants.h:7-8): a = {0, 3/4, 1/3}, b = {1, 1/4, 2/3}
+      if ((rc = st(A, B, {0.0, 1.0, 1.0, 0.0, 0, 0}))) return rc;
+      if ((rc = st(B, C, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, 0.0, 0, 0}))) return rc;
+      if ((rc = st(C, A, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, 0.0, 1, 0}))) return rc;
+      break;
   }
   HIPCHK(launch_finish(c->L));
   return 0;

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(const EnsProb<typename P
     TS* U1 = (TS*)pr.U[1];
     TS* U2 = (TS*)pr.U[2];
     StageArgs<TS> A{stage == 0 ? U0 : stage == 1 ? U1 : U2, U0,
-                    stage == 0 ? U1 : stage == 1 ? U2 : U0, ctl, a, b, last};
+                    stage == 0 ? U1 : stage == 1 ? U2 : U0, ctl, a, b, b, last, nullptr, 0.0, 0};
     TL::load(A, pr.P, pr.g, lds, tid, it.bx, it.by);
     __syncthreads();
     typename PH::TF Fr[TL::ROUNDS][TL::NV];

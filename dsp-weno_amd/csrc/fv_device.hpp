@@ -353,7 +353,48 @@ __device__ __forceinline__ void dsp_rec(const NetView<TW>& net, bool arch5, T u0
 }
 
 // ---------------------------------------------------------------- params
-enum { REC_SPWENO = 0, REC_ENO = 1, REC_DSP = 2, REC_FIRST = 3 };
+enum { REC_SPWENO = 0, REC_ENO = 1, REC_DSP = 2, REC_FIRST = 3, REC_SECOND = 4, REC_MINMOD = 5,
+       REC_MINABS = 6 };
+
+// limited_slope (reconstruct.cc:162-186) with minmod / minabs (limiter.h:20-50)
+template <typename T>
+__device__ __forceinline__ T limited_slope(T a, T b, bool use_minabs) {
+  T m;
+  if (use_minabs) {
+    m = (m_abs(a) > m_abs(b)) ? b : a;
+  } else {
+    m = (a * b > T(0)) ? m_min(m_abs(a), m_abs(b)) * ((a < T(0)) ? T(-1) : T(1)) : T(0);
+  }
+  return T(0.5) * m;
+}
+
+// the two-cell-slope family, l = u_i + s_l, r = u_{i+1} - s_r:
+//   first  (reconstruct.cc:13-24)  s = 0
+//   second (reconstruct.cc:30-50)  unlimited MUSCL, kappa = 1/3; the
+//          reference's KKK macro makes the weights the double constants
+//          1 - 1/3 and 1 + 1/3
+//   minmod / minabs (reconstruct.cc:55-80)  s = limited_slope
+// One body for the four keeps the stage kernels that switch on the scheme
+// at run time small.
+template <typename T>
+__device__ __forceinline__ void muscl(const T* st, int s, int kind, T& l, T& r) {
+  const T u1 = st[s / 2 - 1], u2 = st[s / 2];
+  T sl = T(0), sr = T(0);
+  if (kind != REC_FIRST) {
+    const T d = u2 - u1;
+    const T dl = u1 - st[s / 2 - 2];
+    const T dr = st[s / 2 + 1] - u2;
+    if (kind == REC_SECOND) {
+      sl = (dl * T(1.0 - 1.0 / 3.0) + d * T(1.0 + 1.0 / 3.0)) * T(0.25);
+      sr = (dr * T(1.0 - 1.0 / 3.0) + d * T(1.0 + 1.0 / 3.0)) * T(0.25);
+    } else {
+      sl = limited_slope(dl, d, kind == REC_MINABS);
+      sr = limited_slope(dr, d, kind == REC_MINABS);
+    }
+  }
+  l = u1 + sl;
+  r = u2 - sr;
+}
 
 template <typename T>
 struct Phys {
@@ -379,10 +420,16 @@ struct Phys {
   NetView<T> net;
 };
 
-// reconstruct one component on a stencil of 2*ng values
-template <typename T>
+// reconstruct one component on a stencil of 2*ng values.  RS selects the
+// schemes compiled in: 0 all, 1 all but second/minmod/minabs, 2 only those
+// three (kernels whose register budget is tight instantiate 1 and 2 apart).
+template <int RS = 0, typename T>
 __device__ __forceinline__ void recon_any(const Phys<T>& P, const T* st, int ng, T& l, T& r) {
   const int s = 2 * ng;
+  if (RS == 2 || (RS == 0 && P.recon >= REC_SECOND)) {
+    muscl(st, s, P.recon, l, r);
+    return;
+  }
   switch (P.recon) {
     case REC_ENO:
       eno_rec(P.eno_k, st, ng, l, r);

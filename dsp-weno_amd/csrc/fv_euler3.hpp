@@ -113,7 +113,87 @@ __device__ __forceinline__ void kep3(const Phys<T>& P, const Pr3<T>& l, const Pr
   F[4] = en * area;
 }
 
-template <typename T, int G>
+// keps_flux keps.cc:10-48
+template <typename T>
+__device__ __forceinline__ void keps3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,
+                                      const T n[3], T area, T F[5]) {
+  const T rho = T(0.5) * (l.rho + r.rho);
+  const T templ = l.p / (P.R * l.rho), tempr = r.p / (P.R * r.rho);
+  const T vel2 = T(0.5) * (vdot3(l.u, l.u) + vdot3(r.u, r.u));
+  const T betal = T(0.5) / (P.R * templ), betar = T(0.5) / (P.R * tempr);
+  const T beta = T(0.5) * (betal + betar);
+  const T p = T(0.5) * rho / beta;
+  T vel[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) vel[d] = l.u[d] * T(0.5) + r.u[d] * T(0.5);
+  const T mass = rho * vdot3(vel, n);
+  T mom[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * p + vel[d] * mass;
+  const T en = T(0.5) * (T(1) / ((P.gamma - T(1)) * beta) - vel2) * mass + vdot3(mom, vel);
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+// roe_flux roe.cc:10-86 (no entropy fix)
+template <typename T>
+__device__ __forceinline__ void roe3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,
+                                     const T n[3], T area, T F[5]) {
+  const T hl = P.gamma * l.p / l.rho / (P.gamma - T(1)) + T(0.5) * vdot3(l.u, l.u);
+  const T hr = P.gamma * r.p / r.rho / (P.gamma - T(1)) + T(0.5) * vdot3(r.u, r.u);
+  const T sl = m_sqrt(l.rho), sr = m_sqrt(r.rho);
+  const T fl = sl / (sl + sr), fr = T(1) - fl;
+  const T rho = sl * sr;
+  T vel[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) vel[d] = l.u[d] * fl + r.u[d] * fr;
+  const T h = hl * fl + hr * fr;
+  const T vn = vdot3(vel, n);
+  const T a = m_sqrt((P.gamma - T(1)) * (h - T(0.5) * vdot3(vel, vel)));
+  const T dp = r.p - l.p;
+  const T vln = vdot3(l.u, n), vrn = vdot3(r.u, n);
+  const T dV = vrn - vln;
+  T mass, mom[3], en;
+  if (vn >= T(0)) {
+    const T coeff = T(0.5) * (dp - rho * a * dV) / (a * a);
+    const T factor = m_min(vn - a, T(0)) * coeff;
+    mass = l.rho * vln;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) mom[d] = n[d] * l.p + l.u[d] * (l.rho * vln);
+    en = hl * mass;
+    mass += factor;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) mom[d] += vel[d] * factor + n[d] * (-a * factor);
+    en += (h - a * vn) * factor;
+  } else {
+    const T coeff = T(0.5) * (dp + rho * a * dV) / (a * a);
+    const T factor = m_max(vn + a, T(0)) * coeff;
+    mass = r.rho * vrn;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) mom[d] = n[d] * r.p + r.u[d] * (r.rho * vrn);
+    en = hr * mass;
+    mass -= factor;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) mom[d] += vel[d] * (-factor) + n[d] * (-a * factor);
+    en -= (h + a * vn) * factor;
+  }
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+// FAM 0: the TeCNO and entropy-conservative fluxes (100-106, 120-124) with
+//        sp_weno(c), eno, sp_weno_dl or first;
+// FAM 1: roe, kep4, keps, keps4 (125-128);
+// FAM 2: as FAM 0 with second, minmod or minabs.
+// Separate instantiations keep the face of the common case inlined and its
+// register pressure unchanged by the rarely used options.
+template <typename T, int G, int FAM = 0>
 struct Euler3Ph {
   using TS = T;
   using TF = T;
@@ -178,8 +258,10 @@ struct Euler3Ph {
         kepec3(P, a, b, n, area, out);
       else if (kind == 1)
         roeec3(P, a, b, n, area, out);
-      else
+      else if (kind == 2 || FAM == 0)
         kep3(P, a, b, n, area, out);
+      else
+        keps3(P, a, b, n, area, out);
     };
     auto ec4 = [&](int kind, T out[5]) {  // kepec4.cc:10-38, roe_ec4.cc:10-34
       T f0[5];
@@ -201,6 +283,13 @@ struct Euler3Ph {
     if (fl == 122) { ec(1, q[L], q[R], F); return; }
     if (fl == 123) { ec4(1, F); return; }
     if (fl == 124) { ec(2, q[L], q[R], F); return; }
+    if constexpr (FAM == 1) {
+      if (fl == 125) roe3(P, q[L], q[R], n, area, F);
+      else if (fl == 126) ec4(2, F);
+      else if (fl == 127) ec(3, q[L], q[R], F);
+      else ec4(3, F);  // 128 keps4
+      return;
+    }
     // TeCNO: entropy-conservative part
     if (fl == 101 || fl == 103) ec4(0, F);
     else if (fl == 100 || fl == 102) ec(0, q[L], q[R], F);
@@ -267,7 +356,7 @@ struct Euler3Ph {
                         Rm[4][i] * ld[E3_VL * NB + cc]);
       }
       T zl, zr;
-      recon_any(P, st, G, zl, zr);
+      recon_any<FAM == 2 ? 2 : 1>(P, st, G, zl, zr);
       dZ[i] = zr - zl;
     }
     T Diff[5];
@@ -520,7 +609,7 @@ struct Tile3 {
                                 double dt, T& smax, unsigned& nbad) {
     const int64_t N = g.N;
     const double factor = dt / P.vol_d;
-    const T a = T(A.a), b = T(A.b), fb = T(factor * A.b);
+    const T a = T(A.a), b = T(A.b), fb = T(factor * A.c);
     for (int c = tid; c < NCELL; c += 256) {
       const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
       const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
@@ -554,9 +643,7 @@ struct Tile3 {
         }
         const T u = A.U[v * N + idx];
         const T u0 = A.U0[v * N + idx];
-        T un = u * b;
-        un += (u0 * a - fb * r);
-        A.Uout[v * N + idx] = un;
+        A.Uout[v * N + idx] = rk_update<NV>(A, v * N + idx, u, u0, r, a, b, fb, factor);
       }
       if (A.last) {
         smax = m_max(smax, PH::speed(P, A.Uout, N, idx));

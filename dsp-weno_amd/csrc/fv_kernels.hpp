@@ -36,15 +36,57 @@ struct GridDev {
   int64_t N;
 };
 
+// coefficients of one Runge-Kutta stage (update_solution, enflo/src/fv.cc:1160-1198,
+// enflo_scalar/src_scalar/fv.cc:459-499):
+//   rk1 / ssprk3:  u = b*u + a*u0 - (dt/vol)*c*res, c = b (ext_constants.h:7-8)
+//   jameson_rk4:   a = 1, b = 0, c = jameson_rks[k] (ext_constants.h:9)
+//   rk4:           R2 accumulates w*res with w = 1/rk4_rks[k]; stages 0-2
+//                  take u = u0 - (dt/vol)*rk4_rks[k+1]*res (a = 1, b = 0),
+//                  the last u = u0 - (dt/vol)*R2/6
+struct StageCoef {
+  double a, b, c;
+  double w;   // weight of res in the classical RK4 accumulator
+  int last;   // reduce the CFL wave speed of Uout
+  int r2;     // classical RK4 accumulator: 0 unused, 1 first stage, 2 middle, 3 last
+};
+
 template <typename T>
 struct StageArgs {
   const T* U;    // U^(k-1), NV x N
   const T* U0;   // U^n
   T* Uout;       // U^(k)
   Ctl* ctl;
-  double a, b;   // u <- b*u + a*u0 - (dt/vol)*b*res
-  int last;      // reduce the CFL wave speed of Uout
+  double a, b, c;
+  int last;
+  T* R2;         // classical RK4 residual accumulator, NV x N (r2 != 0)
+  double w;
+  int r2;
 };
+
+// the stage's new value of one variable: r is the cell's residual (the
+// reference's residual[i][j][k] for that variable), factor = dt/vol
+template <int NV, typename TS>
+__device__ __forceinline__ TS rk_update(const StageArgs<TS>& A, int64_t at, TS u, TS u0, TS r,
+                                        TS a, TS b, TS fb, double factor) {
+  if (A.r2 == 0) {
+    TS un = u * b;
+    un += (u0 * a - fb * r);
+    return un;
+  }
+  // residual2 += residual/rk4_rks[rk] (scalar fv.cc:492) and
+  // residual2.sadd(residual, 1/rk4_rks[rk]) (systems fv.cc:1191): the
+  // weights 1 and 2 make both exact
+  TS r2 = r * TS(A.w);
+  if (A.r2 != 1) r2 = A.R2[at] + r2;
+  if (A.r2 != 3) {
+    A.R2[at] = r2;
+    return u0 - fb * r;
+  }
+  // last stage: the scalar solver rounds old - factor*R2/6 (fv.cc:496),
+  // the systems solver old + R2*(-factor/6) (fv.cc:1195)
+  if (NV == 1) return u0 - TS(factor) * r2 / TS(6);
+  return u0 + r2 * TS(-factor / 6.0);
+}
 
 // map a (possibly ghost) coordinate along one axis to a source cell; returns
 // the sign to apply to the normal momentum (-1 for a wall mirror)
@@ -218,7 +260,7 @@ struct Tile {
     const int x0 = bxi * TX, y0 = byi * TY;
     const int64_t N = g.N;
     const double factor = dt / P.vol_d;
-    const TS a = TS(A.a), b = TS(A.b), fb = TS(factor * A.b);
+    const TS a = TS(A.a), b = TS(A.b), fb = TS(factor * A.c);
     for (int c = tid; c < SH::NCELL; c += 256) {
       const int ly = c / TX, lx = c - ly * TX;
       const int gx = x0 + lx, gy = y0 + ly;
@@ -236,9 +278,7 @@ struct Tile {
         }
         const TS u = A.U[v * N + idx];
         const TS u0 = A.U0[v * N + idx];
-        TS un = u * b;
-        un += (u0 * a - fb * r);
-        A.Uout[v * N + idx] = un;
+        A.Uout[v * N + idx] = rk_update<NV>(A, v * N + idx, u, u0, r, a, b, fb, factor);
       }
       if (A.last) {
         smax = m_max(smax, PH::speed(P, A.Uout, N, idx));

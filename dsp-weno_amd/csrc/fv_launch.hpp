@@ -20,6 +20,7 @@ struct LaunchInfo {
   GridDev grid;
   hipStream_t stream;
   Ctl* ctl;
+  void* R2;       // classical RK4 residual accumulator (state type, NV x N) or null
 };
 
 // ---------------------------------------------------------------- control
@@ -329,7 +330,7 @@ __global__ void k_flux_batch(Phys<T> P, int system, const double* Pin, const dou
 template <typename T>
 struct Launch {
   static hipError_t stage(const LaunchInfo& L, const Phys<T>& P, const void* U, const void* U0,
-                          void* Uout, double a, double b, int last);
+                          void* Uout, const StageCoef& k);
   static hipError_t speed(const LaunchInfo& L, const Phys<T>& P, const void* U);
   static hipError_t recon_batch(const LaunchInfo& L, const Phys<T>& P, const double* st, int ng,
                                 int64_t n, double* l, double* r);
@@ -389,10 +390,9 @@ struct StateOps {
 
 template <class PH, int DIMS, int TX, int TY, int G>
 static hipError_t launch_stage_t(const LaunchInfo& L, const Phys<typename PH::TP>& P,
-                                 const void* U, const void* U0, void* Uo, double a, double b,
-                                 int last) {
+                                 const void* U, const void* U0, void* Uo, const StageCoef& k) {
   using TS = typename PH::TS;
-  StageArgs<TS> A{(const TS*)U, (const TS*)U0, (TS*)Uo, L.ctl, a, b, last};
+  StageArgs<TS> A{(const TS*)U, (const TS*)U0, (TS*)Uo, L.ctl, k.a, k.b, k.c, k.last, (TS*)L.R2, k.w, k.r2};
   dim3 grid((L.grid.n[0] + TX - 1) / TX, DIMS >= 2 ? (L.grid.n[1] + TY - 1) / TY : 1, 1);
   hipLaunchKernelGGL((stage_kernel<PH, DIMS, TX, TY, G>), grid, dim3(256), 0, L.stream, A, P,
                      L.grid);
@@ -401,11 +401,10 @@ static hipError_t launch_stage_t(const LaunchInfo& L, const Phys<typename PH::TP
 
 template <class PH, int G, bool VISC>
 static hipError_t launch_stage3_t(const LaunchInfo& L, const Phys<typename PH::TP>& P,
-                                  const void* U, const void* U0, void* Uo, double a, double b,
-                                  int last) {
+                                  const void* U, const void* U0, void* Uo, const StageCoef& k) {
   using TS = typename PH::TS;
   constexpr int TX = 8, TY = 8, TZ = 2;
-  StageArgs<TS> A{(const TS*)U, (const TS*)U0, (TS*)Uo, L.ctl, a, b, last};
+  StageArgs<TS> A{(const TS*)U, (const TS*)U0, (TS*)Uo, L.ctl, k.a, k.b, k.c, k.last, (TS*)L.R2, k.w, k.r2};
   dim3 grid((L.grid.n[0] + TX - 1) / TX, (L.grid.n[1] + TY - 1) / TY, (L.grid.n[2] + TZ - 1) / TZ);
   hipLaunchKernelGGL((stage3_kernel<PH, TX, TY, TZ, G, VISC>), grid, dim3(256), 0, L.stream, A, P,
                      L.grid);
@@ -424,15 +423,15 @@ static hipError_t launch_speed_t(const LaunchInfo& L, const Phys<typename PH::TP
 
 template <typename T>
 hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U, const void* U0,
-                            void* Uo, double a, double b, int last) {
+                            void* Uo, const StageCoef& k) {
   constexpr int X2 = TECNO_TILE2_X, Y2 = TECNO_TILE2_Y, X1 = TECNO_TILE1_X;
   if (L.system == 0) {
     if (L.dims == 1)
-      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 1, X1, 1, 3>(L, P, U, U0, Uo, a, b, last)
-                      : launch_stage_t<ScalarPh<T, 2>, 1, X1, 1, 2>(L, P, U, U0, Uo, a, b, last);
+      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 1, X1, 1, 3>(L, P, U, U0, Uo, k)
+                      : launch_stage_t<ScalarPh<T, 2>, 1, X1, 1, 2>(L, P, U, U0, Uo, k);
     if (L.dims == 2)
-      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 2, X2, Y2, 3>(L, P, U, U0, Uo, a, b, last)
-                      : launch_stage_t<ScalarPh<T, 2>, 2, X2, Y2, 2>(L, P, U, U0, Uo, a, b, last);
+      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 2, X2, Y2, 3>(L, P, U, U0, Uo, k)
+                      : launch_stage_t<ScalarPh<T, 2>, 2, X2, Y2, 2>(L, P, U, U0, Uo, k);
     return hipErrorNotSupported;
   }
   if (L.dims == 3) {
@@ -440,8 +439,16 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
     if constexpr (sizeof(T) == 8) {
       const bool visc = P.model == 11 || P.model == 12;
       if (L.G != 2) return hipErrorNotSupported;
-      if (visc) return launch_stage3_t<Euler3Ph<T, 2>, 2, true>(L, P, U, U0, Uo, a, b, last);
-      return launch_stage3_t<Euler3Ph<T, 2>, 2, false>(L, P, U, U0, Uo, a, b, last);
+      if (P.flux >= 125) {
+        if (visc) return launch_stage3_t<Euler3Ph<T, 2, 1>, 2, true>(L, P, U, U0, Uo, k);
+        return launch_stage3_t<Euler3Ph<T, 2, 1>, 2, false>(L, P, U, U0, Uo, k);
+      }
+      if (P.recon >= REC_SECOND) {
+        if (visc) return launch_stage3_t<Euler3Ph<T, 2, 2>, 2, true>(L, P, U, U0, Uo, k);
+        return launch_stage3_t<Euler3Ph<T, 2, 2>, 2, false>(L, P, U, U0, Uo, k);
+      }
+      if (visc) return launch_stage3_t<Euler3Ph<T, 2>, 2, true>(L, P, U, U0, Uo, k);
+      return launch_stage3_t<Euler3Ph<T, 2>, 2, false>(L, P, U, U0, Uo, k);
     }
     return hipErrorNotSupported;
   }
@@ -452,26 +459,25 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
       return hipErrorNotSupported;
     constexpr int XM = TECNO_TILE2M_X, YM = TECNO_TILE2M_Y;
     if (L.fast_rec == RV_SPWENO)
-      return launch_stage_t<Euler2Mixed<RV_SPWENO, MM_SIMT>, 2, XM, YM, 2>(L, P, U, U0, Uo, a, b,
-                                                                          last);
+      return launch_stage_t<Euler2Mixed<RV_SPWENO, MM_SIMT>, 2, XM, YM, 2>(L, P, U, U0, Uo, k);
     if (L.mlp_mode == MM_BF16)
-      return launch_stage_t<Euler2Mixed<RV_DSP, MM_BF16>, 2, XM, YM, 2>(L, P, U, U0, Uo, a, b, last);
+      return launch_stage_t<Euler2Mixed<RV_DSP, MM_BF16>, 2, XM, YM, 2>(L, P, U, U0, Uo, k);
     if (L.mlp_mode == MM_FP16)
-      return launch_stage_t<Euler2Mixed<RV_DSP, MM_FP16>, 2, XM, YM, 2>(L, P, U, U0, Uo, a, b, last);
-    return launch_stage_t<Euler2Mixed<RV_DSP, MM_SIMT>, 2, XM, YM, 2>(L, P, U, U0, Uo, a, b, last);
+      return launch_stage_t<Euler2Mixed<RV_DSP, MM_FP16>, 2, XM, YM, 2>(L, P, U, U0, Uo, k);
+    return launch_stage_t<Euler2Mixed<RV_DSP, MM_SIMT>, 2, XM, YM, 2>(L, P, U, U0, Uo, k);
   } else {
     if (L.G == 3)
       return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 3>(L, P, U, U0, Uo,
-                                                                                 a, b, last);
+                                                                                 k);
     constexpr int XD = TECNO_TILE2D_X, YD = TECNO_TILE2D_Y;
     if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_SPWENO)
       return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_SPWENO>, 2, XD, YD, 2>(L, P, U, U0,
-                                                                                    Uo, a, b, last);
+                                                                                    Uo, k);
     if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_DSP)
       return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_DSP>, 2, XD, YD, 2>(L, P, U, U0, Uo,
-                                                                                 a, b, last);
+                                                                                 k);
     return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
-                                                                               a, b, last);
+                                                                               k);
   }
 }
 

@@ -169,6 +169,73 @@ __device__ __forceinline__ void kep2(const Phys<T>& P, T rl, T nl, T tl, T pl, T
   F[3] = en * area;
 }
 
+// KEPS flux (keps.cc:10-48) in the face frame: arithmetic density, mean of
+// the two betas
+template <typename T>
+__device__ __forceinline__ void keps2(const Phys<T>& P, T rl, T nl, T tl, T pl, T rr, T nr, T tr,
+                                      T pr, T area, T F[4]) {
+  const T rho = T(0.5) * (rl + rr);
+  const T templ = pl / (P.R * rl), tempr = pr / (P.R * rr);
+  const T vel2 = T(0.5) * ((nl * nl + tl * tl) + (nr * nr + tr * tr));
+  const T betal = T(0.5) / (P.R * templ), betar = T(0.5) / (P.R * tempr);
+  const T beta = T(0.5) * (betal + betar);
+  const T p = T(0.5) * rho / beta;
+  const T vn = nl * T(0.5) + nr * T(0.5);
+  const T vt = tl * T(0.5) + tr * T(0.5);
+  const T mass = rho * vn;
+  const T mn = p + vn * mass;
+  const T mt = vt * mass;
+  const T en = T(0.5) * (T(1) / (P.gm1 * beta) - vel2) * mass + (mn * vn + mt * vt);
+  F[0] = mass * area;
+  F[1] = mn * area;
+  F[2] = mt * area;
+  F[3] = en * area;
+}
+
+// Roe's flux without entropy fix (roe.cc:10-86) in the face frame: the
+// upwind cell's flux plus the one wave that crosses the face
+template <typename T>
+__device__ __forceinline__ void roe2(const Phys<T>& P, T rl, T nl, T tl, T pl, T rr, T nr, T tr,
+                                     T pr, T area, T F[4]) {
+  const T hl = P.gamma * pl / rl / P.gm1 + T(0.5) * (nl * nl + tl * tl);
+  const T hr = P.gamma * pr / rr / P.gm1 + T(0.5) * (nr * nr + tr * tr);
+  const T sl = m_sqrt(rl), sr = m_sqrt(rr);
+  const T fl = sl / (sl + sr), fr = T(1) - fl;
+  const T rho = sl * sr;
+  const T vn = nl * fl + nr * fr, vt = tl * fl + tr * fr;
+  const T h = hl * fl + hr * fr;
+  const T a = m_sqrt(P.gm1 * (h - T(0.5) * (vn * vn + vt * vt)));
+  const T dp = pr - pl, dV = nr - nl;
+  T mass, mn, mt, en;
+  if (vn >= T(0)) {
+    const T coeff = T(0.5) * (dp - rho * a * dV) / (a * a);
+    const T factor = m_min(vn - a, T(0)) * coeff;
+    mass = rl * nl;
+    mn = pl + nl * (rl * nl);
+    mt = tl * (rl * nl);
+    en = hl * mass;
+    mass += factor;
+    mn += vn * factor + (-a * factor);
+    mt += vt * factor;
+    en += (h - a * vn) * factor;
+  } else {
+    const T coeff = T(0.5) * (dp + rho * a * dV) / (a * a);
+    const T factor = m_max(vn + a, T(0)) * coeff;
+    mass = rr * nr;
+    mn = pr + nr * (rr * nr);
+    mt = tr * (rr * nr);
+    en = hr * mass;
+    mass -= factor;
+    mn += vn * (-factor) + (-a * factor);
+    mt += vt * (-factor);
+    en -= (h + a * vn) * factor;
+  }
+  F[0] = mass * area;
+  F[1] = mn * area;
+  F[2] = mt * area;
+  F[3] = en * area;
+}
+
 enum { FV_KEPES4_ROE = 0, FV_GENERIC = 1 };
 enum { RV_SPWENO = 0, RV_DSP = 1, RV_GENERIC = 2 };
 
@@ -236,6 +303,40 @@ struct Euler2Ph {
     const T area = P.area[ax];
     const int L = G - 1, R = G;  // centre pair
     const int fl = (FV == FV_KEPES4_ROE) ? 101 : P.flux;
+    if (FV == FV_GENERIC && fl >= 125) {
+      // fluxes without dissipation on reconstructed jumps: roe, kep4, keps,
+      // keps4 (num_flux.cc:12-72); the 4th-order forms combine three
+      // two-point fluxes as kep4.cc / keps4.cc
+      T Fe[4];
+      auto pl_ = [&](int l) { return lds[E_P * NB + base + l * stride]; };
+      if (fl == 125) {
+        roe2(P, rho[L], un[L], ut[L], pl_(L), rho[R], un[R], ut[R], pl_(R), area, Fe);
+      } else if (fl == 127) {
+        keps2(P, rho[L], un[L], ut[L], pl_(L), rho[R], un[R], ut[R], pl_(R), area, Fe);
+      } else {
+        auto two = [&](int a, int b, T f[4]) {
+          if (fl == 126)
+            kep2(P, rho[a], un[a], ut[a], pl_(a), rho[b], un[b], ut[b], pl_(b), area, f);
+          else
+            keps2(P, rho[a], un[a], ut[a], pl_(a), rho[b], un[b], ut[b], pl_(b), area, f);
+        };
+        T f0[4];
+        two(L, R, f0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Fe[v] = T(0) + f0[v] * T(4.0 / 3.0);
+        two(L - 1, R, f0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Fe[v] += f0[v] * T(-1.0 / 6.0);
+        two(L, R + 1, f0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Fe[v] += f0[v] * T(-1.0 / 6.0);
+      }
+      F[0] = Fe[0];
+      F[1] = ax ? Fe[2] : Fe[1];  // face frame (n, t) -> (x, y)
+      F[2] = ax ? Fe[1] : Fe[2];
+      F[3] = Fe[3];
+      return;
+    }
     T Fe[4], rhoh, betah;
     const bool pure_ec = fl >= 120;
     // ---- entropy-conservative part

@@ -41,10 +41,6 @@ struct Net5 {
       A[l] = __builtin_bit_cast(V, w);
     }
   }
-  __device__ static V pack(float x0, float x1, float x2, float x3, float x4) {
-    V v = {(E)x0, (E)x1, (E)x2, (E)x3, (E)x4, (E)1.0f, (E)0.0f, (E)0.0f};
-    return v;
-  }
   __device__ static f32x16 mma(V a, V x, f32x16 c) {
     if constexpr (H)
       return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, x, c, 0, 0, 0);
@@ -56,6 +52,13 @@ struct Net5 {
     typedef float f32x2 __attribute__((ext_vector_type(2)));
     typedef E e16x2 __attribute__((ext_vector_type(2)));
     return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, e16x2));
+  }
+  // network input: five features, the constant-1 bias input at 5, zeros
+  // (one packed conversion per two values)
+  __device__ static V pack(float x0, float x1, float x2, float x3, float x4) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 w = {cvt2(x0, x1), cvt2(x2, x3), cvt2(x4, 1.0f), 0u};
+    return __builtin_bit_cast(V, w);
   }
   // ReLU after the conversion: a bf16 / fp16 value is negative exactly when
   // its bit pattern is a negative 16-bit integer, so max(bits, 0) as packed

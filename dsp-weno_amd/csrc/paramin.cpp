@@ -369,7 +369,9 @@ void read_numeric(Reader& r, Param& p) {
   const std::string ts = r.token();
   if (ts == "ssprk3") c.time_scheme = TECNO_TS_SSPRK3;
   else if (ts == "rk1") c.time_scheme = TECNO_TS_RK1;
-  else throw InputError("time_scheme " + ts + " is not provided by this build (ssprk3, rk1)");
+  else if (ts == "jameson_rk4") c.time_scheme = TECNO_TS_JAMESON_RK4;
+  else if (ts == "rk4") c.time_scheme = TECNO_TS_RK4;
+  else throw InputError("unknown time_scheme " + ts);
   r.entry("cfl");
   c.cfl = r.number();
   r.entry("max_iter");
@@ -388,6 +390,9 @@ void read_numeric(Reader& r, Param& p) {
   else if (rec == "sp_weno_corr2") { c.recon = TECNO_REC_SP_WENO; c.spweno_corr = 2; }
   else if (rec == "sp_weno_dl") c.recon = TECNO_REC_SP_WENO_DL;
   else if (rec == "first") c.recon = TECNO_REC_FIRST;
+  else if (rec == "second") c.recon = TECNO_REC_SECOND;
+  else if (rec == "minmod") c.recon = TECNO_REC_MINMOD;
+  else if (rec == "minabs") c.recon = TECNO_REC_MINABS;
   else if (rec == "eno") {
     c.recon = TECNO_REC_ENO;
     c.eno_level = r.integer();
@@ -470,7 +475,8 @@ void read_material(Reader& r, Param& p) {
         {"kepes_tecno_roe", 100}, {"kepes_tecno4_roe", 101}, {"kepes_tecno_rusanov", 102},
         {"kepes_tecno4_rusanov", 103}, {"roe_tecno_roe", 104}, {"roe_tecno4_roe", 105},
         {"kep_tecno_roe", 106}, {"kepec", 120}, {"kepec4", 121}, {"roe_ec", 122},
-        {"roe_ec4", 123}, {"kep", 124}};
+        {"roe_ec4", 123}, {"kep", 124}, {"roe", 125}, {"kep4", 126}, {"keps", 127},
+        {"keps4", 128}};
     c.flux = -1;
     for (auto& nm : names)
       if (f == nm.first) c.flux = nm.second;

@@ -243,7 +243,8 @@ int main(int argc, char** argv) {
       std::cout << std::setprecision(17);
       std::cout << "system " << (p.scalar ? "scalar" : "euler") << "\nflux " << p.base.flux
                 << "\nmodel " << p.base.model << "\nrecon " << p.base.recon << " "
-                << p.base.spweno_corr << " " << p.base.eno_level << "\nreconstruction "
+                << p.base.spweno_corr << " " << p.base.eno_level << "\ntime_scheme "
+                << p.base.time_scheme << "\nreconstruction "
                 << p.reconstruction << "\nlevels " << p.Nx.size() << "\nfinal_time "
                 << p.final_time << "\nt_stamps " << p.t_stamps << "\nooc " << p.ooc
                 << "\npdim " << p.pdim << "\ncfl " << p.base.cfl << "\ngamma " << p.base.gamma

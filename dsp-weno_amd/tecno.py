@@ -21,10 +21,12 @@ SCALAR_FLUX = {"lin_ec": 0, "lin_ec4": 1, "lin_tecno": 2, "lin_tecno4": 3,
 EULER_FLUX = {"kepes_tecno_roe": 100, "kepes_tecno4_roe": 101, "kepes_tecno_rusanov": 102,
               "kepes_tecno4_rusanov": 103, "roe_tecno_roe": 104, "roe_tecno4_roe": 105,
               "kep_tecno_roe": 106, "kepec": 120, "kepec4": 121, "roe_ec": 122,
-              "roe_ec4": 123, "kep": 124}
+              "roe_ec4": 123, "kep": 124, "roe": 125, "kep4": 126, "keps": 127, "keps4": 128}
 REC_SP_WENO, REC_ENO, REC_SP_WENO_DL, REC_FIRST = 0, 1, 2, 3
+REC_SECOND, REC_MINMOD, REC_MINABS = 4, 5, 6
 BC = {"periodic": 0, "dirichlet": 1, "neumann": 2, "wall": 3}
-TS_RK1, TS_SSPRK3 = 0, 1
+TS_RK1, TS_SSPRK3, TS_JAMESON_RK4, TS_RK4 = 0, 1, 2, 3
+TIME_SCHEME = {"rk1": TS_RK1, "ssprk3": TS_SSPRK3, "jameson_rk4": TS_JAMESON_RK4, "rk4": TS_RK4}
 FP64, FP32 = 0, 1
 MLP_FP64, MLP_FP32, MLP_BF16, MLP_FP16 = 0, 1, 2, 3
 
@@ -45,6 +47,12 @@ def recon_from_name(name):
         return REC_ENO, 0, int(toks[1])
     if toks[0] == "first":
         return REC_FIRST, 0, 2
+    if toks[0] == "second":
+        return REC_SECOND, 0, 2
+    if toks[0] == "minmod":
+        return REC_MINMOD, 0, 2
+    if toks[0] == "minabs":
+        return REC_MINABS, 0, 2
     raise ValueError(f"unknown reconstruction {name!r}")
 
 

@@ -63,19 +63,27 @@ enum { TECNO_FLUX_KEPES_TECNO_ROE = 100, TECNO_FLUX_KEPES_TECNO4_ROE,
        TECNO_FLUX_ROE_TECNO_ROE, TECNO_FLUX_ROE_TECNO4_ROE,
        TECNO_FLUX_KEP_TECNO_ROE,
        TECNO_FLUX_KEPEC = 120, TECNO_FLUX_KEPEC4, TECNO_FLUX_ROE_EC,
-       TECNO_FLUX_ROE_EC4, TECNO_FLUX_KEP };
+       TECNO_FLUX_ROE_EC4, TECNO_FLUX_KEP,
+       /* fluxes without entropy dissipation on the reconstructed jumps:
+        * Roe without entropy fix (roe.cc:10-86), KEP 4th order (kep4.cc),
+        * KEPS and KEPS 4th order (keps.cc, keps4.cc)                        */
+       TECNO_FLUX_ROE = 125, TECNO_FLUX_KEP4, TECNO_FLUX_KEPS, TECNO_FLUX_KEPS4 };
 
 /* reconstruction: enflo/src/parameter.cc:155-212.  sp_wenoc and
- * sp_weno_corr2 are TECNO_REC_SP_WENO with spweno_corr = 1 / 2.           */
+ * sp_weno_corr2 are TECNO_REC_SP_WENO with spweno_corr = 1 / 2.  second:
+ * unlimited MUSCL (kappa = 1/3), minmod / minabs: MUSCL with the limited
+ * slope (reconstruct.cc:30-80, 162-186).                                  */
 enum { TECNO_REC_SP_WENO = 0, TECNO_REC_ENO = 1, TECNO_REC_SP_WENO_DL = 2,
-       TECNO_REC_FIRST = 3 };
+       TECNO_REC_FIRST = 3, TECNO_REC_SECOND = 4, TECNO_REC_MINMOD = 5,
+       TECNO_REC_MINABS = 6 };
 
 /* boundary types: enflo/src/ext_constants.h:23 (same values)               */
 enum { TECNO_BC_PERIODIC = 0, TECNO_BC_DIRICHLET = 1, TECNO_BC_NEUMANN = 2,
        TECNO_BC_WALL = 3 };
 
-/* time schemes: enflo/src/ext_constants.h:20 (ssprk3 and rk1 supported)    */
-enum { TECNO_TS_RK1 = 0, TECNO_TS_SSPRK3 = 1 };
+/* time schemes: enflo/src/ext_constants.h:20 (same values).  rk4 holds one
+ * more state-sized device buffer, the accumulated residual.                */
+enum { TECNO_TS_RK1 = 0, TECNO_TS_SSPRK3 = 1, TECNO_TS_JAMESON_RK4 = 2, TECNO_TS_RK4 = 3 };
 
 /* arithmetic of the face kernels */
 enum { TECNO_FP64 = 0, TECNO_FP32 = 1 };

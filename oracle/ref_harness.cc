@@ -96,6 +96,20 @@ static void dump_state(FV& fv, FILE* f) {
 }
 #endif
 
+// classical RK4 clears its accumulated residual at the start of every step
+// (fv.cc:1267-1274; enflo_scalar fv.cc:558-566)
+static void clear_residual2(FV& fv) {
+  if (fv.param.time_scheme != rk4) return;
+  for (int i = 0; i < fv.Nx_cell; ++i)
+    for (int j = 0; j < fv.Ny_cell; ++j)
+      for (int k = 0; k < fv.Nz_cell; ++k)
+#ifdef HARNESS_SCALAR
+        fv.residual2[i][j][k] = 0.0;
+#else
+        fv.residual2[i][j][k].zero();
+#endif
+}
+
 static int mode_run(char* param, int nsteps, const char* out) {
   FV fv(param);
   setup(fv);
@@ -120,6 +134,7 @@ static int mode_run(char* param, int nsteps, const char* out) {
 #endif
     fv.compute_dt();
     if (fv.time + fv.dt > t_save) fv.dt = t_save - fv.time;
+    clear_residual2(fv);
     for (int rk = 0; rk < fv.param.n_rks; ++rk) {
 #ifdef HARNESS_SCALAR
       fv.compute_residual(rk);
@@ -168,6 +183,7 @@ static int mode_time(char* param, int nsteps) {
 #endif
     fv.compute_dt();
     if (fv.time + fv.dt > t_save) fv.dt = t_save - fv.time;
+    clear_residual2(fv);
     for (int rk = 0; rk < fv.param.n_rks; ++rk) {
 #ifdef HARNESS_SCALAR
       fv.compute_residual(rk);

@@ -375,6 +375,18 @@ void tor_nn_forward(const tor_net* net, const double in[4], double* recl,
   *recr = wt[3] * 0.5 * (in[1] + in[2]) + wt[2] * (-0.5 * in[3] + 1.5 * in[2]);
 }
 
+/* minmod (limiter.h:20-35): min(|a|,|b|)*SIGN(a) if a*b > 0, else 0;
+ * minabs (limiter.h:40-52): the argument of smaller magnitude, a on ties;
+ * limited_slope (reconstruct.cc:162-186) halves either */
+static double minmod(double a, double b) {
+  if (a * b > 0.0) return dmin(fabs(a), fabs(b)) * ((a < 0) ? -1 : 1);
+  return 0.0;
+}
+static double minabs(double a, double b) { return (fabs(a) > fabs(b)) ? b : a; }
+double tor_limited_slope(double a, double b, int use_minabs) {
+  return 0.5 * (use_minabs ? minabs(a, b) : minmod(a, b));
+}
+
 /* reconstruct dispatch: enflo/src/reconstruct.cc:115-157, 192-239 */
 void tor_reconstruct(const tecno_cfg* cfg, const tor_net* net,
                      const double* st, int ng, double* l, double* r) {
@@ -394,6 +406,27 @@ void tor_reconstruct(const tecno_cfg* cfg, const tor_net* net,
       } else {
         tor_nn_forward(net, c, l, r);
       }
+      break;
+    }
+    case TECNO_REC_SECOND: {
+      /* reconstruct_second_muscl, reconstruct.cc:30-50; KKK is the macro
+       * 1.0/3.0, so the weights are (1.0-1.0/3.0) and (1.0+1.0/3.0) */
+      double d = st[s / 2] - st[s / 2 - 1];
+      double dl = st[s / 2 - 1] - st[s / 2 - 2];
+      double dr = st[s / 2 + 1] - st[s / 2];
+      *l = st[s / 2 - 1] + (dl * (1.0 - 1.0 / 3.0) + d * (1.0 + 1.0 / 3.0)) * 0.25;
+      *r = st[s / 2] - (dr * (1.0 - 1.0 / 3.0) + d * (1.0 + 1.0 / 3.0)) * 0.25;
+      break;
+    }
+    case TECNO_REC_MINMOD:
+    case TECNO_REC_MINABS: {
+      /* reconstruct_muscl_limited, reconstruct.cc:55-80 */
+      int mabs = cfg->recon == TECNO_REC_MINABS;
+      double d = st[s / 2] - st[s / 2 - 1];
+      double dl = st[s / 2 - 1] - st[s / 2 - 2];
+      double dr = st[s / 2 + 1] - st[s / 2];
+      *l = st[s / 2 - 1] + tor_limited_slope(dl, d, mabs);
+      *r = st[s / 2] - tor_limited_slope(dr, d, mabs);
       break;
     }
     case TECNO_REC_FIRST:
@@ -566,10 +599,85 @@ static void kep_flux(const tecno_cfg* c, const prim_t* Pl, const prim_t* Pr,
   F[4] = en * area;
 }
 
+/* keps_flux: enflo/src/keps.cc:10-48 (arithmetic density, mean beta) */
+static void keps_flux(const tecno_cfg* c, const prim_t* Pl, const prim_t* Pr,
+                      const double n[3], double area, double F[5]) {
+  double R = c->gas_const, G = c->gamma;
+  double rhol = Pl->rho, rhor = Pr->rho;
+  double rho = 0.5 * (rhol + rhor);
+  double templ = Pl->p / (R * rhol);
+  double tempr = Pr->p / (R * rhor);
+  double vel2 = 0.5 * (vsq(Pl->u) + vsq(Pr->u));
+  double betal = 0.5 / (R * templ);
+  double betar = 0.5 / (R * tempr);
+  double beta = 0.5 * (betal + betar);
+  double p = 0.5 * rho / beta;
+  double vel[3];
+  for (int d = 0; d < 3; ++d) vel[d] = Pl->u[d] * 0.5 + Pr->u[d] * 0.5;
+  double vn = vdot(vel, n);
+  double mass = rho * vn;
+  double mom[3];
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * p + vel[d] * mass;
+  double en = 0.5 * (1.0 / ((G - 1.0) * beta) - vel2) * mass + vdot(mom, vel);
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
+/* roe_flux: enflo/src/roe.cc:10-86, Roe's flux without entropy fix: the
+ * upwind cell's physical flux plus the one acoustic wave that crosses */
+static void roe_flux(const tecno_cfg* c, const prim_t* Pl, const prim_t* Pr,
+                     const double n[3], double area, double F[5]) {
+  double G = c->gamma;
+  double rhol = Pl->rho, rhor = Pr->rho, pl = Pl->p, pr = Pr->p;
+  double hl = G * pl / rhol / (G - 1.0) + 0.5 * vsq(Pl->u);
+  double hr = G * pr / rhor / (G - 1.0) + 0.5 * vsq(Pr->u);
+  double sl = sqrt(rhol), sr = sqrt(rhor);
+  double fl = sl / (sl + sr);
+  double fr = 1.0 - fl;
+  double rho = sl * sr;
+  double vel[3];
+  for (int d = 0; d < 3; ++d) vel[d] = Pl->u[d] * fl + Pr->u[d] * fr;
+  double h = hl * fl + hr * fr;
+  double vn = vdot(vel, n);
+  double a = sqrt((G - 1.0) * (h - 0.5 * vsq(vel)));
+  double dp = pr - pl;
+  double vln = vdot(Pl->u, n), vrn = vdot(Pr->u, n);
+  double dV = vrn - vln;
+  double mass, mom[3], en;
+  if (vn >= 0.0) {
+    double coeff = 0.5 * (dp - rho * a * dV) / (a * a);
+    double factor = dmin(vn - a, 0.0) * coeff;
+    mass = rhol * vln;
+    for (int d = 0; d < 3; ++d) mom[d] = n[d] * pl + Pl->u[d] * (rhol * vln);
+    en = hl * mass;
+    mass += factor;
+    for (int d = 0; d < 3; ++d) mom[d] += vel[d] * factor + n[d] * (-a * factor);
+    en += (h - a * vn) * factor;
+  } else {
+    double coeff = 0.5 * (dp + rho * a * dV) / (a * a);
+    double factor = dmax(vn + a, 0.0) * coeff;
+    mass = rhor * vrn;
+    for (int d = 0; d < 3; ++d) mom[d] = n[d] * pr + Pr->u[d] * (rhor * vrn);
+    en = hr * mass;
+    mass -= factor;
+    for (int d = 0; d < 3; ++d) mom[d] += vel[d] * (-factor) + n[d] * (-a * factor);
+    en -= (h + a * vn) * factor;
+  }
+  F[0] = mass * area;
+  F[1] = mom[0] * area;
+  F[2] = mom[1] * area;
+  F[3] = mom[2] * area;
+  F[4] = en * area;
+}
+
 typedef void (*ec_fn)(const tecno_cfg*, const prim_t*, const prim_t*,
                       const double*, double, double*);
 
-/* 4th-order combination: kepec4.cc:10-38, roe_ec4.cc:10-34 (Flux::sadd) */
+/* 4th-order combination: kepec4.cc:10-38, roe_ec4.cc:10-34, kep4.cc:10-38,
+ * keps4.cc:10-38 (Flux::sadd) */
 static void ec4_flux(ec_fn fn, const tecno_cfg* c, const prim_t* P, int ng,
                      const double n[3], double area, double F[5]) {
   int s = 2 * ng;
@@ -601,6 +709,10 @@ static void euler_face_flux(const tecno_cfg* c, const tor_net* net,
     case TECNO_FLUX_ROE_EC: roe_ec_flux(c, &P[s / 2 - 1], &P[s / 2], n, area, F); return;
     case TECNO_FLUX_ROE_EC4: ec4_flux(roe_ec_flux, c, P, ng, n, area, F); return;
     case TECNO_FLUX_KEP: kep_flux(c, &P[s / 2 - 1], &P[s / 2], n, area, F); return;
+    case TECNO_FLUX_ROE: roe_flux(c, &P[s / 2 - 1], &P[s / 2], n, area, F); return;
+    case TECNO_FLUX_KEP4: ec4_flux(kep_flux, c, P, ng, n, area, F); return;
+    case TECNO_FLUX_KEPS: keps_flux(c, &P[s / 2 - 1], &P[s / 2], n, area, F); return;
+    case TECNO_FLUX_KEPS4: ec4_flux(keps_flux, c, P, ng, n, area, F); return;
     default: break;
   }
   double R = c->gas_const, G = c->gamma;
@@ -806,6 +918,7 @@ struct tor_solver {
   double* U;
   double* Uo;
   double* res;
+  double* res2;   /* classical RK4: accumulated residual (residual2, fv.h:221) */
   double* Fface;  /* face flux scratch, (ncell + extra) x nvar per axis */
   double time;
   int iter;
@@ -825,7 +938,8 @@ static int flux_ng(int flux) {
     case TECNO_FLUX_LIN_EC4: case TECNO_FLUX_LIN_TECNO4: case TECNO_FLUX_BUR_EC4:
     case TECNO_FLUX_BUR_TECNO4: case TECNO_FLUX_KEPES_TECNO4_ROE:
     case TECNO_FLUX_KEPES_TECNO4_RUSANOV: case TECNO_FLUX_ROE_TECNO4_ROE:
-    case TECNO_FLUX_KEPEC4: case TECNO_FLUX_ROE_EC4:
+    case TECNO_FLUX_KEPEC4: case TECNO_FLUX_ROE_EC4: case TECNO_FLUX_KEP4:
+    case TECNO_FLUX_KEPS4:
       return 2;
     default:
       return 1;
@@ -883,6 +997,7 @@ tor_solver* tor_create(const tecno_cfg* cfg, const tor_net* net, int faithful) {
     s->Uo = (double*)calloc(s->ncell * 5, sizeof(double));
   }
   s->res = (double*)calloc(s->ncell * s->nvar, sizeof(double));
+  s->res2 = (double*)calloc(s->ncell * s->nvar, sizeof(double));
   return s;
 }
 
@@ -895,6 +1010,7 @@ void tor_destroy(tor_solver* s) {
   free(s->U);
   free(s->Uo);
   free(s->res);
+  free(s->res2);
   free(s->Fface);
   free(s);
 }
@@ -1335,24 +1451,41 @@ static void visc_residual(tor_solver* s) {
 
 static const double ARK[3] = {0.0, 3.0 / 4.0, 1.0 / 3.0}; /* ext_constants.h:7 */
 static const double BRK[3] = {1.0, 1.0 / 4.0, 2.0 / 3.0}; /* ext_constants.h:8 */
+static const double JRK[4] = {1.0 / 4.0, 1.0 / 3.0, 1.0 / 2.0, 1.0}; /* ext_constants.h:9 */
+static const double RK4C[4] = {1.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};      /* ext_constants.h:10 */
 
 int tor_step(tor_solver* s, double t_stamp, double* dt_out) {
   double dt = compute_dt(s);
   if (s->time + dt > t_stamp) dt = t_stamp - s->time; /* fv.cc:1266 */
   double factor = dt / (s->g.h[0] * s->g.h[1] * s->g.h[2]);
-  int nrk = (s->c.time_scheme == TECNO_TS_RK1) ? 1 : 3;
+  int ts = s->c.time_scheme;
+  /* n_rks: parameter.cc:111-132 */
+  int nrk = ts == TECNO_TS_RK1 ? 1 : (ts == TECNO_TS_SSPRK3 ? 3 : 4);
   int64_t N = s->ncell;
+  if (ts == TECNO_TS_RK4) /* residual2 zeroed per step, fv.cc:1267-1274 */
+    memset(s->res2, 0, (size_t)s->nvar * N * sizeof(double));
   if (s->nvar == 1) {
     memcpy(s->uold, s->u, s->npad * sizeof(double));
     for (int rk = 0; rk < nrk; ++rk) {
       scalar_residual(s);
-      /* update_solution: enflo_scalar fv.cc:463-476 */
+      /* update_solution: enflo_scalar fv.cc:459-499 */
       for (int k = 0; k < s->n[2]; ++k)
         for (int j = 0; j < s->n[1]; ++j)
           for (int i = 0; i < s->n[0]; ++i) {
             int64_t p = pidx(s, i, j, k);
-            s->u[p] *= BRK[rk];
-            s->u[p] += (s->uold[p] * ARK[rk] - factor * BRK[rk] * s->res[iidx(s, i, j, k)]);
+            int64_t q = iidx(s, i, j, k);
+            if (ts == TECNO_TS_JAMESON_RK4) {
+              s->u[p] = (s->uold[p] - factor * JRK[rk] * s->res[q]);
+            } else if (ts == TECNO_TS_RK4) {
+              s->res2[q] += (s->res[q] / RK4C[rk]);
+              if (rk < 3)
+                s->u[p] = (s->uold[p] - factor * RK4C[rk + 1] * s->res[q]);
+              else
+                s->u[p] = (s->uold[p] - factor * s->res2[q] / 6.0);
+            } else {
+              s->u[p] *= BRK[rk];
+              s->u[p] += (s->uold[p] * ARK[rk] - factor * BRK[rk] * s->res[q]);
+            }
           }
       scalar_bc(s, s->u);
     }
@@ -1363,13 +1496,28 @@ int tor_step(tor_solver* s, double t_stamp, double* dt_out) {
       euler_prim_to_ent_all(s);
       euler_residual(s);
       if (is_ns(&s->c)) visc_residual(s); /* compute_residual fv.cc:591-603 */
-      /* update_solution: enflo fv.cc:1163-1175 (ConVar::sadd, convar.h:118) */
-      double c2 = -factor * BRK[rk];
-      for (int64_t q = 0; q < N; ++q)
-        for (int v = 0; v < 5; ++v) {
-          s->U[5 * q + v] *= BRK[rk];
-          s->U[5 * q + v] += s->Uo[5 * q + v] * ARK[rk] + s->res[5 * q + v] * c2;
+      /* update_solution: enflo fv.cc:1160-1198 (ConVar::sadd convar.h:118,
+       * ConVar::equ convar.h:103, Flux::sadd flux.h:127) */
+      if (ts == TECNO_TS_JAMESON_RK4) {
+        double c2 = -factor * JRK[rk];
+        for (int64_t q = 0; q < 5 * N; ++q) s->U[q] = s->Uo[q] * 1.0 + s->res[q] * c2;
+      } else if (ts == TECNO_TS_RK4) {
+        double w = 1.0 / RK4C[rk];
+        for (int64_t q = 0; q < 5 * N; ++q) {
+          s->res2[q] += s->res[q] * w;
+          if (rk < 3)
+            s->U[q] = s->Uo[q] * 1.0 + s->res[q] * (-factor * RK4C[rk + 1]);
+          else
+            s->U[q] = s->Uo[q] * 1.0 + s->res2[q] * (-factor / 6.0);
         }
+      } else {
+        double c2 = -factor * BRK[rk];
+        for (int64_t q = 0; q < N; ++q)
+          for (int v = 0; v < 5; ++v) {
+            s->U[5 * q + v] *= BRK[rk];
+            s->U[5 * q + v] += s->Uo[5 * q + v] * ARK[rk] + s->res[5 * q + v] * c2;
+          }
+      }
       /* con_to_prim: fv.cc:388-400 */
       for (int k = 0; k < s->n[2]; ++k)
         for (int j = 0; j < s->n[1]; ++j)

@@ -34,6 +34,8 @@ typedef struct tor_net {
 double tor_logavg(double a, double b);
 void tor_sp_weno(const double u[4], int corr_type, double* l, double* r);
 double tor_eno_pw(int k, const double* s);
+/* 0.5*minmod(a, b) or 0.5*minabs(a, b): reconstruct.cc:162-186, limiter.h:20-52 */
+double tor_limited_slope(double a, double b, int use_minabs);
 void tor_eno_rec(int k, const double* st, int ng, double* l, double* r);
 int tor_net_load(const char* path, tor_net* net);
 int tor_net_save(const char* path, const tor_net* net);

@@ -5,7 +5,7 @@ import os
 
 import numpy as np
 
-from tecno import make_cfg
+from tecno import TIME_SCHEME, make_cfg
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
@@ -35,6 +35,11 @@ def scalar_cfg(key, **kw):
     if key.startswith("sc1d_") and key[5:] in recs:
         c = make_cfg(n=(64, 2, 2), recon=recs[key[5:]], flux="lin_tecno4", **SC1D, **kw)
         return c, 0.5, 5
+    if key.startswith("sc1d_rec_"):
+        return make_cfg(n=(64, 2, 2), recon=key[9:], flux="lin_tecno4", **SC1D, **kw), 0.5, 5
+    if key.startswith("sc1d_ts_"):
+        return make_cfg(n=(64, 2, 2), recon="sp_weno", flux="lin_tecno4",
+                        time_scheme=TIME_SCHEME[key[8:]], **SC1D, **kw), 0.5, 5
     if key == "config1_spweno":
         return make_cfg(n=(400, 2, 2), recon="sp_weno", flux="lin_tecno4", **SC1D, **kw), 0.5, 5
     if key.startswith("sc1d_"):
@@ -60,7 +65,12 @@ def scalar_cfg(key, **kw):
 
 def euler_cfg(key, **kw):
     """tecno_cfg of the Euler golden case `key` (fixtures_euler.npz)."""
-    recs = {"spweno": "sp_weno", "spwenoc": "sp_wenoc", "eno3": "eno 3", "dsp": "sp_weno_dl"}
+    recs = {"spweno": "sp_weno", "spwenoc": "sp_wenoc", "eno3": "eno 3", "dsp": "sp_weno_dl",
+            "first": "first", "second": "second", "minmod": "minmod", "minabs": "minabs"}
+    if key.startswith("eu2d_ts_"):
+        return make_cfg(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(20, 16, 2), dims=2,
+                        recon="sp_weno", flux="kepes_tecno4_roe", bc="neumann",
+                        time_scheme=TIME_SCHEME[key[8:]], **kw), 0.25, 5
     if key.startswith("eu2d_riemann_"):
         return make_cfg(lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(20, 16, 2), dims=2,
                         recon=recs[key[13:]], flux="kepes_tecno4_roe", bc="neumann", **kw), 0.25, 5
@@ -76,6 +86,21 @@ def euler_cfg(key, **kw):
     if key == "eu3d_tg":
         return make_cfg(lo=(0.0, 0.0, 0.0), hi=(2 * math.pi,) * 3, n=(8, 6, 6), dims=3, recon="sp_weno",
                         flux="kepes_tecno4_roe", bc="periodic", **kw), 1.0, 5
+    if key.startswith("eu3d_tg_"):
+        tag = key[8:]
+        flux, rec, ts = "kepes_tecno4_roe", "sp_weno", "ssprk3"
+        if tag in ("roe", "kep4", "keps", "keps4"):
+            flux = tag
+        elif tag in TIME_SCHEME:
+            ts = tag
+        else:
+            rec = tag
+        return make_cfg(lo=(0.0, 0.0, 0.0), hi=(2 * math.pi,) * 3, n=(8, 6, 6), dims=3, recon=rec,
+                        flux=flux, bc="periodic", time_scheme=TIME_SCHEME[ts], **kw), 1.0, 5
+    if key == "ns3d_es_rk4":
+        return make_cfg(lo=(0.0, 0.0, 0.0), hi=(2 * math.pi,) * 3, n=(10, 8, 6), dims=3, recon="minabs",
+                        flux="kepes_tecno4_roe", bc="periodic", model="ns_es",
+                        viscosity=("constant", 1.0 / 1600), time_scheme=TIME_SCHEME["rk4"], **kw), 1.0, 5
     if key.startswith("ns3d_"):
         spec = {"ns3d_es_tg": ("ns_es", ("constant", 1.0 / 1600), "sp_weno", (10, 8, 6)),
                 "ns3d_kep_tg": ("ns_kep", ("constant", 1.0 / 1600), "sp_weno", (8, 10, 6)),

@@ -32,13 +32,13 @@ def param_in(*, system, lo, hi, n, recon, flux, final_time, ic, bc="periodic",
              cfl=0.5, t_stamps=5, pdim=1, vel=(1, 0, 0), model=None,
              constants=(), net=NET_FIXTURE, ec1_fix="no", out=".",
              max_iter=50000, exact=None, ooc="no", euler_model="euler",
-             viscosity=("constant", 0.0), prandtl=0.72):
+             viscosity=("constant", 0.0), prandtl=0.72, time_scheme="ssprk3"):
     """Text of a param.in file (sections in the order Parameter::read needs)."""
     L = ["grid", "{"]
     for d, ax in enumerate("xyz"):
         L += [f"   {ax}_min   {lo[d]!r}", f"   {ax}_max   {hi[d]!r}",
               f"   N{ax}_cell {{ {', '.join(str(v) for v in np.atleast_1d(n[d]))} }}"]
-    L += ["}", "", "numeric", "{", "   time_scheme  ssprk3", f"   cfl          {cfl!r}",
+    L += ["}", "", "numeric", "{", f"   time_scheme  {time_scheme}", f"   cfl          {cfl!r}",
           f"   max_iter     {max_iter}", f"   final_time   {final_time!r}",
           "   min_residue  1.0e-6", f"   reconstruct  {recon}", f"   net_file     {net}", "}", ""]
     L += ["material", "{"]
@@ -260,6 +260,19 @@ def scalar_cases(tmp):
                     ic="sin(2*pi*x)*cos(2*pi*y)*sin(2*pi*z) + 0.3*cos(2*pi*(x+z))", pdim=3, out=tmp)
     init, t, dt, st = harness_run(REFH_SCA, text, 4, tmp)
     fx["sc3d_lin"] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
+    # the MUSCL-family reconstructions and the other time schemes, 1D
+    for rec in ["first", "second", "minmod", "minabs"]:
+        text = param_in(n=(64, 2, 2), recon=rec, flux="lin_tecno4", **base)
+        init, t, dt, st = harness_run(REFH_SCA, text, 40, tmp)
+        ns = len(t)
+        fx[f"sc1d_rec_{rec}"] = dict(init=init, times=t, dts=dt, states=st[[0, 4, ns - 1]],
+                                     steps=np.array([1, 5, ns]))
+    for ts in ["rk1", "jameson_rk4", "rk4"]:
+        text = param_in(n=(64, 2, 2), recon="sp_weno", flux="lin_tecno4", time_scheme=ts, **base)
+        init, t, dt, st = harness_run(REFH_SCA, text, 40, tmp)
+        ns = len(t)
+        fx[f"sc1d_ts_{ts}"] = dict(init=init, times=t, dts=dt, states=st[[0, 4, ns - 1]],
+                                   steps=np.array([1, 5, ns]))
     return fx
 
 
@@ -360,6 +373,44 @@ def euler_cases(tmp):
                     bc=["wall", "wall", "neumann", "wall", "periodic", "periodic"], pdim=2, out=tmp)
     init, t, dt, st = harness_run(REFH_SYS, text, 4, tmp)
     fx["eu2d_wall"] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
+    # fluxes without dissipation on reconstructed jumps, on the vortex
+    for flux in ["roe", "kep4", "keps", "keps4"]:
+        text = param_in(system="euler", lo=(-5.0, -5.0, 0.0), hi=(5.0, 5.0, 1.0), n=(20, 20, 2),
+                        recon="sp_weno", flux=flux, final_time=10.0, ic=v_ic, constants=vc,
+                        bc="periodic", pdim=2, out=tmp)
+        init, t, dt, st = harness_run(REFH_SYS, text, 3, tmp)
+        fx[f"eu2d_vortex_{flux}"] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
+    # MUSCL-family reconstructions on the quadrant Riemann problem
+    for rec in ["first", "second", "minmod", "minabs"]:
+        text = param_in(system="euler", lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(20, 16, 2),
+                        recon=rec, flux="kepes_tecno4_roe", final_time=0.25, ic=ic,
+                        constants=consts, bc="neumann", pdim=2, out=tmp)
+        init, t, dt, st = harness_run(REFH_SYS, text, 6, tmp)
+        fx[f"eu2d_riemann_{rec}"] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
+    # time schemes on the quadrant Riemann problem
+    for ts in ["rk1", "jameson_rk4", "rk4"]:
+        text = param_in(system="euler", lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(20, 16, 2),
+                        recon="sp_weno", flux="kepes_tecno4_roe", final_time=0.25, ic=ic,
+                        constants=consts, bc="neumann", pdim=2, out=tmp, time_scheme=ts)
+        init, t, dt, st = harness_run(REFH_SYS, text, 6, tmp)
+        fx[f"eu2d_ts_{ts}"] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
+    # 3D: the extra fluxes, reconstructions and time schemes on the
+    # Taylor-Green-like flow, and classical RK4 on a viscous case
+    for tag, flux, rec, ts in [("roe", "roe", "sp_weno", "ssprk3"), ("keps4", "keps4", "sp_weno", "ssprk3"),
+                               ("kep4", "kep4", "sp_weno", "ssprk3"), ("keps", "keps", "sp_weno", "ssprk3"),
+                               ("minmod", "kepes_tecno4_roe", "minmod", "ssprk3"),
+                               ("second", "kepes_tecno4_roe", "second", "ssprk3"),
+                               ("jameson_rk4", "kepes_tecno4_roe", "sp_weno", "jameson_rk4")]:
+        text = param_in(system="euler", lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, n=(8, 6, 6),
+                        recon=rec, flux=flux, final_time=1.0, ic=tg, pdim=3, bc="periodic", out=tmp,
+                        time_scheme=ts)
+        init, t, dt, st = harness_run(REFH_SYS, text, 3, tmp)
+        fx[f"eu3d_tg_{tag}"] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
+    text = param_in(system="euler", lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, n=(10, 8, 6), recon="minabs",
+                    flux="kepes_tecno4_roe", final_time=1.0, ic=tgv, pdim=3, bc="periodic", out=tmp,
+                    euler_model="ns_es", viscosity=("constant", 1.0 / 1600), time_scheme="rk4")
+    init, t, dt, st = harness_run(REFH_SYS, text, 3, tmp)
+    fx["ns3d_es_rk4"] = dict(init=init, times=t, dts=dt, states=st[[0, len(t) - 1]], steps=np.array([1, len(t)]))
     return fx
 
 
@@ -374,6 +425,11 @@ def point_fixtures(tmp):
                         recon=rec, flux="lin_tecno4", final_time=0.1, ic="x", out=tmp)
         lr = harness_recon(REFH_SCA, text, st, 2, tmp)
         fx[f"recon_{tag}"] = dict(l=lr[:, 0], r=lr[:, 1])
+    for rec in ["first", "second", "minmod", "minabs"]:
+        text = param_in(system="scalar", lo=(-1.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(8, 2, 2),
+                        recon=rec, flux="lin_tecno4", final_time=0.1, ic="x", out=tmp)
+        lr = harness_recon(REFH_SCA, text, st, 2, tmp)
+        fx[f"recon_{rec}"] = dict(l=lr[:, 0], r=lr[:, 1])
     # ENO (stencils of 2k values)
     for k in (2, 3):
         base4 = recon_stencils(rng, 4000)
@@ -393,6 +449,13 @@ def point_fixtures(tmp):
                         final_time=0.1, ic="x", out=tmp)
         outs = [harness_flux(REFH_SCA, text, s1, ax, 1, tmp)[:, 0] for ax in range(3)]
         fx[f"sflux_{flux}"] = dict(F=np.stack(outs))
+    for flux, model in [("lin_tecno4", None), ("bur_tecno", "burgers")]:
+        for rec in ["second", "minmod", "minabs"]:
+            text = param_in(system="scalar", lo=(-1.0, -2.0, 0.0), hi=(1.0, 1.0, 0.5), n=(8, 6, 4),
+                            recon=rec, flux=flux, model=model, vel=(1.25, -0.5, 0.75),
+                            final_time=0.1, ic="x", out=tmp)
+            outs = [harness_flux(REFH_SCA, text, s1, ax, 1, tmp)[:, 0] for ax in range(3)]
+            fx[f"sfluxrec_{flux}_{rec}"] = dict(F=np.stack(outs))
     # Euler fluxes on random stencils (entropy variables from prim_to_ent)
     ent_text = param_in(system="euler", lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), n=(4, 4, 4),
                         recon="sp_weno", flux="kepes_tecno4_roe", final_time=0.1,
@@ -405,9 +468,10 @@ def point_fixtures(tmp):
         data = np.concatenate([P, V], axis=-1)
         for flux in ["kepes_tecno4_roe", "kepes_tecno_roe", "kepes_tecno4_rusanov", "kepes_tecno_rusanov",
                      "roe_tecno4_roe", "roe_tecno_roe", "kep_tecno_roe", "kepec", "kepec4", "roe_ec",
-                     "roe_ec4", "kep"]:
+                     "roe_ec4", "kep", "roe", "kep4", "keps", "keps4"]:
             recs = [("sp_weno", "spweno")] if flux != "kepes_tecno4_roe" else \
-                [("sp_weno", "spweno"), ("sp_wenoc", "spwenoc"), ("sp_weno_dl", "dsp")]
+                [("sp_weno", "spweno"), ("sp_wenoc", "spwenoc"), ("sp_weno_dl", "dsp"),
+                 ("first", "first"), ("second", "second"), ("minmod", "minmod"), ("minabs", "minabs")]
             for rec, rtag in recs:
                 for ec1 in (["no", "yes"] if flux == "kepes_tecno4_roe" and rec == "sp_weno" else ["no"]):
                     text = param_in(system="euler", lo=(0.0, 0.0, 0.0), hi=(1.0, 2.0, 0.5), n=(8, 6, 4),

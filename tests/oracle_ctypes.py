@@ -29,6 +29,8 @@ def lib():
         P = C.c_void_p
         L.tor_logavg.restype = C.c_double
         L.tor_logavg.argtypes = [C.c_double, C.c_double]
+        L.tor_limited_slope.restype = C.c_double
+        L.tor_limited_slope.argtypes = [C.c_double, C.c_double, C.c_int]
         L.tor_net_load.argtypes = [C.c_char_p, C.POINTER(Net)]
         L.tor_net_save.argtypes = [C.c_char_p, C.POINTER(Net)]
         L.tor_sp_weno_batch.argtypes = [C.c_int, dp, C.c_int64, dp, dp]
@@ -107,6 +109,10 @@ def logavg(a, b):
     out = np.zeros_like(a)
     lib().tor_logavg_batch(_dp(a), _dp(b), a.size, _dp(out))
     return out
+
+
+def limited_slope(a, b, use_minabs):
+    return lib().tor_limited_slope(float(a), float(b), int(bool(use_minabs)))
 
 
 def sp_weno(st, corr=0):

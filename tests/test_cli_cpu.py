@@ -143,8 +143,8 @@ def test_expression_semantics(expr, value):
 
 
 @pytest.mark.parametrize("kw,msg", [
-    (dict(flux="keps4"), "unknown scalar flux"), (dict(ts="rk4"), "time_scheme"),
-    (dict(rec="minmod"), "reconstruct minmod"), (dict(bc="dirichlet"), "dirichlet"),
+    (dict(flux="keps4"), "unknown scalar flux"), (dict(ts="rk5"), "time_scheme"),
+    (dict(rec="weno5"), "reconstruct weno5"), (dict(bc="dirichlet"), "dirichlet"),
     (dict(expr="sin(x"), "missing )"), (dict(expr="foo(x)"), "unknown function"),
     (dict(expr="x + q"), "unknown name q")])
 def test_errors_are_reported(kw, msg):
@@ -153,6 +153,20 @@ def test_errors_are_reported(kw, msg):
     expr = args.pop("expr")
     r = scalar_value(expr, **args)
     assert r.returncode != 0 and msg in r.stderr, (r.returncode, r.stderr)
+
+
+@pytest.mark.parametrize("kw,field,value", [
+    (dict(ts="rk1"), "time_scheme", "0"), (dict(ts="jameson_rk4"), "time_scheme", "2"),
+    (dict(ts="rk4"), "time_scheme", "3"), (dict(rec="first"), "recon", "3 0 2"),
+    (dict(rec="second"), "recon", "4 0 2"), (dict(rec="minmod"), "recon", "5 0 2"),
+    (dict(rec="minabs"), "recon", "6 0 2")])
+def test_numeric_section_names(kw, field, value):
+    """Every time_scheme and reconstruct name of parameter.cc:111-212 maps
+    to its tecno_cfg value."""
+    r = scalar_value("x", **kw)
+    assert r.returncode == 0, r.stderr
+    line = [l for l in r.stdout.splitlines() if l.startswith(field + " ")][0]
+    assert line == f"{field} {value}", line
 
 
 def test_missing_file():

@@ -27,7 +27,9 @@ def net():
 
 # ---------------------------------------------------------------- point level
 @pytest.mark.parametrize("tag,rec", [("spweno", "sp_weno"), ("spwenoc", "sp_wenoc"),
-                                     ("spwenocorr2", "sp_weno_corr2"), ("dsp", "sp_weno_dl")])
+                                     ("spwenocorr2", "sp_weno_corr2"), ("dsp", "sp_weno_dl"),
+                                     ("first", "first"), ("second", "second"),
+                                     ("minmod", "minmod"), ("minabs", "minabs")])
 def test_reconstruct_matches_reference(tag, rec):
     st = case("point", "recon_stencils")["st"]
     ref = case("point", f"recon_{tag}")
@@ -58,6 +60,29 @@ def test_scalar_flux_matches_reference(flux):
         np.testing.assert_array_equal(orc.flux(cfg, st, None, ax, h)[:, 0], F[ax])
 
 
+@pytest.mark.parametrize("flux", ["lin_tecno4", "bur_tecno"])
+@pytest.mark.parametrize("rec", ["second", "minmod", "minabs"])
+def test_scalar_flux_with_muscl_matches_reference(flux, rec):
+    st = case("point", "sflux_stencils")["st"]
+    F = case("point", f"sfluxrec_{flux}_{rec}")["F"]
+    cfg = make_cfg(system="scalar", flux=flux, recon=rec, n=(8, 6, 4), lo=(-1, -2, 0),
+                   hi=(1, 1, 0.5), vel=(1.25, -0.5, 0.75))
+    h = [2 / 8, 3 / 6, 0.5 / 4]
+    for ax in range(3):
+        np.testing.assert_array_equal(orc.flux(cfg, st, None, ax, h)[:, 0], F[ax])
+
+
+def test_limited_slopes_known_values():
+    """minmod / minabs on hand-worked pairs (limiter.h:20-52): equal signs
+    take the smaller magnitude, opposite signs give 0 for minmod and the
+    smaller-magnitude argument for minabs, ties keep the first argument."""
+    cases = [(1.0, 3.0, 0.5, 0.5), (-4.0, -2.0, -1.0, -1.0), (2.0, -1.0, 0.0, -0.5),
+             (-1.0, 0.5, 0.0, 0.25), (0.0, 5.0, 0.0, 0.0), (2.0, -2.0, 0.0, 1.0)]
+    for a, b, mm, ma in cases:
+        assert orc.limited_slope(a, b, False) == mm
+        assert orc.limited_slope(a, b, True) == ma
+
+
 def test_prim_to_ent_matches_reference():
     cfg = make_cfg()
     for tag in ("2d", "3d"):
@@ -77,7 +102,8 @@ def test_euler_flux_matches_reference(key):
     ec1 = key.endswith("_ec1")
     rtag = parts[-2] if ec1 else parts[-1]
     flux = "_".join(parts[2:-2] if ec1 else parts[2:-1])
-    rec = {"spweno": "sp_weno", "spwenoc": "sp_wenoc", "dsp": "sp_weno_dl"}[rtag]
+    rec = {"spweno": "sp_weno", "spwenoc": "sp_wenoc", "dsp": "sp_weno_dl", "first": "first",
+           "second": "second", "minmod": "minmod", "minabs": "minabs"}[rtag]
     d = case("point", f"ent_{dim}")
     F = case("point", key)["F"]
     cfg = make_cfg(flux=flux, recon=rec, ec1_fix=int(ec1))

@@ -8,6 +8,7 @@ import tempfile
 import numpy as np
 import pytest
 
+import line_extract as LE
 from cases import euler_cfg, euler_prim_for, case
 from tecno import Solver
 
@@ -57,10 +58,13 @@ def test_kh_files_and_state():
         assert files == [f"sol{k:04d}.vtk" for k in range(6)]
         glob = [l for l in open(os.path.join(out, "globals.dat")).read().splitlines() if l.strip()]
         text = open(os.path.join(out, "sol0005.vtk")).read()
+        line, _ = LE.extract(os.path.join(out, "sol0005.vtk"), 0.0, 1.0, 24)
     assert "DIMENSIONS 25 25 3" in text and "CELL_DATA  1152" in text
     dens = text.split("SCALARS density float 1\nLOOKUP_TABLE default\n")[1].split("VECTORS")[0].split()
     rho_file = np.array([float(v) for v in dens]).reshape(2, 24, 24)
     assert np.array_equal(rho_file[0], rho_file[1])
+    # line_extract along y = z = 0.5: cell row 12 of the upper layer
+    assert np.array_equal(line[:, 1], rho_file[1, 12])
     cfg, final, stamps = euler_cfg("eu2d_kh_spweno")
     s = Solver(cfg)
     s.set_state(euler_prim_for(case("euler", "eu2d_kh_spweno")["init"], 2))

@@ -115,6 +115,7 @@ static void fill_phys(const tecno_ctx* c, Phys<T>& P) {
   P.ec1_fix = g.ec1_fix;
   P.fix_zmom = g.fix_zmom;
   P.sign_fallback = g.sign_fallback;
+  P.fix_edges = g.fix_edges;
   P.mlp_mode = g.mlp_dtype;
   P.gamma = T(g.gamma);
   P.R = T(g.gas_const);
@@ -297,9 +298,9 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   const bool ns = g.model == TECNO_MODEL_NS_ES || g.model == TECNO_MODEL_NS_KEP;
   if (g.system == TECNO_SYS_EULER && ns) {
     if (g.dims != 3) return fail(TECNO_EINVAL, "Navier-Stokes runs on 3D grids");
-    for (int i = 0; i < 6; ++i)
-      if (g.bc[i] != TECNO_BC_PERIODIC)
-        return fail(TECNO_EINVAL, "Navier-Stokes: periodic boundaries only (parameter.cc:564-568)");
+    // neumann and wall boundaries extend the reference (parameter.cc:564-568
+    // allows periodic only): the viscous nodes on them read the same ghost
+    // cells as the faces, composed per axis at edges and corners
     for (int d = 0; d < 3; ++d)
       if (g.n[d] < 2) return fail(TECNO_EINVAL, "Navier-Stokes: at least 2 cells per direction");
     if (!(g.prandtl > 0.0)) return fail(TECNO_EINVAL, "prandtl must be positive");
@@ -325,6 +326,9 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   for (int i = 0; i < 6; ++i)
     if (g.bc[i] == TECNO_BC_DIRICHLET || g.bc[i] < 0 || g.bc[i] > 3)
       return fail(TECNO_EINVAL, "boundary type must be periodic, neumann or wall");
+  for (int d = 0; d < 3; ++d)  // parameter.cc:570-581
+    if ((g.bc[2 * d] == TECNO_BC_PERIODIC) != (g.bc[2 * d + 1] == TECNO_BC_PERIODIC))
+      return fail(TECNO_EINVAL, "periodic boundaries must be paired");
   for (int d = 0; d < 3; ++d) {
     if (g.n[d] < 1) return fail(TECNO_EINVAL, "cell counts must be positive");
     if (!(g.hi[d] > g.lo[d])) return fail(TECNO_EINVAL, "empty domain");

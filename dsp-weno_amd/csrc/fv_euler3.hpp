@@ -579,7 +579,9 @@ struct Tile3 {
   }
 
   // node (a, b, q) of the tile lies between cells a-1 and a (local) along x;
-  // nodes on two global boundaries count twice, as in compute_visc_residual
+  // a node on an edge of two periodic boundaries counts twice, as in
+  // compute_visc_residual's boundary loops (fv.cc:821-1156), unless
+  // fix_edges; nodes on neumann / wall boundaries count once
   __device__ static void node(const Phys<T>& P, const GridDev& g, const T* ld, int tid, int x0,
                               int y0, int z0, T X[4], T Y[4], T Z[4]) {
     const int nx = tid % (TX + 1), ny = (tid / (TX + 1)) % (TY + 1), nz = tid / ((TX + 1) * (TY + 1));
@@ -592,9 +594,11 @@ struct Tile3 {
         for (int q = 0; q < 2; ++q) c[a][b][q] = box(nx - 1 + a + G, ny - 1 + b + G, nz - 1 + q + G);
     PH::node(P, ld, NB, c, X, Y, Z);
     const int gx = x0 + nx - 1, gy = y0 + ny - 1, gz = z0 + nz - 1;
-    const int nb = (gx == -1 || gx == g.n[0] - 1) + (gy == -1 || gy == g.n[1] - 1) +
-                   (gz == -1 || gz == g.n[2] - 1);
-    if (nb == 2) {
+    const bool ox = gx == -1 || gx == g.n[0] - 1, oy = gy == -1 || gy == g.n[1] - 1,
+               oz = gz == -1 || gz == g.n[2] - 1;
+    const int nb = ox + oy + oz;
+    const int np = (ox && g.bc[0] == 0) + (oy && g.bc[2] == 0) + (oz && g.bc[4] == 0);
+    if (nb == 2 && np == 2 && !P.fix_edges) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         X[v] *= T(2);

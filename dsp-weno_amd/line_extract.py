@@ -11,10 +11,13 @@ the reference, output.cc:10-90) are parsed here directly.
 * Without a file argument every `sol*.vtk` of the working directory is read.
 * Probe points: Nx + 1 points x0 + (i/Nx)(x1 - x0) at y = z = 0.5 (the
   line source of the reference, resolution Nx); the first Nx are written.
-* Cell data are not interpolated: a point takes the value of the cell that
-  contains it (a point on a face takes the upper cell, the domain's upper
-  boundary the last cell); a point outside the grid gets 0, as the
-  reference's probe filter gives.
+* Cell data are not interpolated: a point takes the value of the cell
+  floor((x - origin)/spacing) that contains it (a point exactly on a face
+  takes the upper cell, the domain's upper boundary the last cell); a point
+  outside the grid gets 0, as the reference's probe filter gives.  The
+  solver writes SPACING with 6 significant digits, so a probe point on a face
+  can land in the lower cell (1/24 is written 0.0416667) — the same cell the
+  reference's VTK probe picks from the same file.
 * Output: `<file stem>.txt` (numpy.savetxt) with columns x, density,
   x-velocity, pressure for systems files, x, u for scalar files (field
   `solution`); and `<file stem>.pdf` with the same plots when matplotlib is

@@ -66,7 +66,8 @@ class Cfg(C.Structure):
         ("gamma", C.c_double), ("gas_const", C.c_double), ("prandtl", C.c_double),
         ("mu_model", C.c_int32), ("mu_ref", C.c_double), ("T_ref", C.c_double),
         ("T_0", C.c_double), ("omega", C.c_double), ("ec1_fix", C.c_int32),
-        ("fix_zmom", C.c_int32), ("sign_fallback", C.c_int32), ("reserved", C.c_int32 * 8),
+        ("fix_zmom", C.c_int32), ("sign_fallback", C.c_int32), ("fix_edges", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -76,7 +77,7 @@ MU_CONSTANT, MU_SUTHERLAND, MU_POWER = 0, 1, 2
 def make_cfg(*, system="euler", flux="kepes_tecno4_roe", recon="sp_weno", n=(32, 32, 2),
              lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dims=2, bc="periodic", cfl=0.5,
              vel=(1.0, 0.0, 0.0), model=None, precision=FP64, mlp_dtype=MLP_FP64,
-             gamma=1.4, gas_const=1.0, ec1_fix=0, fix_zmom=0, sign_fallback=0,
+             gamma=1.4, gas_const=1.0, ec1_fix=0, fix_zmom=0, sign_fallback=0, fix_edges=0,
              time_scheme=TS_SSPRK3, viscosity=("constant", 0.0), prandtl=0.72):
     """viscosity: ("constant", mu_ref) | ("sutherland", mu_ref, T_ref, T_0) |
     ("power", mu_ref, T_ref, omega), as param.in's material/viscosity block"""
@@ -118,6 +119,7 @@ def make_cfg(*, system="euler", flux="kepes_tecno4_roe", recon="sp_weno", n=(32,
     c.ec1_fix = int(ec1_fix)
     c.fix_zmom = int(fix_zmom)
     c.sign_fallback = int(sign_fallback)
+    c.fix_edges = int(fix_edges)
     return c
 
 

@@ -122,7 +122,12 @@ typedef struct tecno_cfg {
                             the reference does; 1: corrected                */
   int32_t sign_fallback; /* sp_weno_dl: replace a face component by SP-WENO
                             where the sign property fails                   */
-  int32_t reserved[8];
+  int32_t fix_edges;     /* Navier-Stokes: 0 adds the viscous flux of a node
+                            on an edge of two periodic boundaries twice, as
+                            the reference's boundary loops do
+                            (fv.cc:821-1156); 1 once.  Nodes on a neumann
+                            or wall boundary count once either way.        */
+  int32_t reserved[7];
 } tecno_cfg;
 
 typedef struct tecno_ctx tecno_ctx;

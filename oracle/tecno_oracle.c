@@ -1100,16 +1100,34 @@ static void euler_bc(tor_solver* s) {
     }
   }
   if (is_ns(&s->c)) {
-    /* edges and corners, periodic images (fv.cc:317-359; the viscous node
-     * operators read one layer of them) */
+    /* edges and corners: periodic images (fv.cc:317-359; the viscous node
+     * operators read one layer of them).  With neumann or wall boundaries,
+     * which the reference does not allow for Navier-Stokes, each out-of-range
+     * coordinate is mapped as the faces map it (copy, or mirror with the
+     * normal velocity negated), composed over the axes. */
     for (int k = -1; k <= s->n[2]; ++k)
       for (int j = -1; j <= s->n[1]; ++j)
         for (int i = -1; i <= s->n[0]; ++i) {
-          int out = (i < 0 || i >= s->n[0]) + (j < 0 || j >= s->n[1]) + (k < 0 || k >= s->n[2]);
+          int c[3] = {i, j, k};
+          int out = 0, flip[3] = {0, 0, 0};
+          for (int d = 0; d < 3; ++d) {
+            int nd = s->n[d];
+            if (c[d] >= 0 && c[d] < nd) continue;
+            ++out;
+            int bc = s->c.bc[2 * d + (c[d] >= nd)];
+            if (bc == TECNO_BC_PERIODIC)
+              c[d] = (c[d] + nd) % nd;
+            else if (bc == TECNO_BC_WALL) {
+              c[d] = c[d] < 0 ? -1 - c[d] : 2 * nd - 1 - c[d];
+              flip[d] = 1;
+            } else
+              c[d] = c[d] < 0 ? 0 : nd - 1;
+          }
           if (out < 2) continue;
-          int si = (i + s->n[0]) % s->n[0], sj = (j + s->n[1]) % s->n[1],
-              sk = (k + s->n[2]) % s->n[2];
-          s->P[pidx(s, i, j, k)] = s->P[pidx(s, si, sj, sk)];
+          prim_t p = s->P[pidx(s, c[0], c[1], c[2])];
+          for (int d = 0; d < 3; ++d)
+            if (flip[d]) p.u[d] = -p.u[d];
+          s->P[pidx(s, i, j, k)] = p;
         }
   }
 }
@@ -1433,9 +1451,17 @@ static void visc_residual(tor_solver* s) {
           node_flux_es(s, i, j, k, &X, &Y, &Z);
         else
           node_flux_ke(s, i, j, k, &X, &Y, &Z);
-        const int nb = (i == -1 || i == s->n[0] - 1) + (j == -1 || j == s->n[1] - 1) +
-                       (k == -1 || k == s->n[2] - 1);
-        for (int rep = 0; rep < (nb == 2 ? 2 : 1); ++rep)
+        /* a node on an edge of two periodic boundaries is added twice by the
+         * reference's boundary loops (fv.cc:821-1156; fix_edges: once);
+         * nodes on neumann / wall boundaries once */
+        const int ox = i == -1 || i == s->n[0] - 1, oy = j == -1 || j == s->n[1] - 1,
+                  oz = k == -1 || k == s->n[2] - 1;
+        const int nb = ox + oy + oz;
+        const int np = (ox && s->c.bc[0] == TECNO_BC_PERIODIC) +
+                       (oy && s->c.bc[2] == TECNO_BC_PERIODIC) +
+                       (oz && s->c.bc[4] == TECNO_BC_PERIODIC);
+        const int twice = nb == 2 && np == 2 && !s->c.fix_edges;
+        for (int rep = 0; rep < (twice ? 2 : 1); ++rep)
         for (int l = 0; l < 2; ++l)
           for (int m = 0; m < 2; ++m)
             for (int q = 0; q < 2; ++q) {

@@ -138,3 +138,46 @@ def euler_prim_for(state, dims):
         assert np.all(s[3] == 0.0)
         s = s[[0, 1, 2, 4]]
     return s.reshape(s.shape[0], -1)
+
+
+# ---------------------------------------------------------------- NS boundaries
+def ns_box_cfg(n, hi, bc, **kw):
+    """3D Navier-Stokes (ns_es, mu = 1/1600) on [0, hi] with kepes_tecno4_roe +
+    sp_weno and the z-momentum fix."""
+    return make_cfg(lo=(0.0, 0.0, 0.0), hi=hi, n=n, dims=3, recon="sp_weno", flux="kepes_tecno4_roe",
+                    bc=bc, model="ns_es", viscosity=("constant", 1.0 / 1600), fix_zmom=1, **kw)
+
+
+def tgv_sub_box(m, kz, rng, z_uniform=False):
+    """(5, kz, m, m) primitive state on the cell centres of [0, pi]^2 x [0, kz*h]
+    (h = pi/m): Taylor-Green velocity and pressure plus seeded noise, so that no
+    symmetry other than the mirror planes of the test holds."""
+    h = math.pi / m
+    x = (np.arange(m) + 0.5) * h
+    z = (np.arange(kz) + 0.5) * h
+    Z, Y, X = np.meshgrid(z, x, x, indexing="ij")
+    cz = np.ones_like(Z) if z_uniform else np.cos(Z)
+    u = np.sin(X) * np.cos(Y) * cz
+    v = -np.cos(X) * np.sin(Y) * cz
+    w = np.zeros_like(X) if z_uniform else 0.3 * np.sin(Z) * np.cos(X)
+    p = 10.0 + 0.0625 * (np.cos(2 * X) + np.cos(2 * Y)) * ((np.ones_like(Z) if z_uniform else np.cos(2 * Z)) + 2)
+    rho = 1.0 + 0.1 * np.sin(X + 2 * Y)
+    s = np.stack([rho, u, v, w, p])
+    noise = rng.normal(size=s.shape) * 0.01
+    if z_uniform:
+        noise[:] = noise[:, :1]
+        noise[3] = 0.0
+    return s + noise
+
+
+def mirror_full(sub, axes):
+    """Periodic state of twice the extent along `axes` (0 = x, 1 = y, 2 = z of
+    a (5, nz, ny, nx) array) whose mirror planes at 0 and the sub-box edge
+    hold `sub` and its images with the normal velocity negated."""
+    s = np.asarray(sub)
+    for d in axes:
+        ax = 3 - d  # array axis of direction d
+        img = np.flip(s, axis=ax).copy()
+        img[1 + d] = -img[1 + d]
+        s = np.concatenate([s, img], axis=ax)
+    return s

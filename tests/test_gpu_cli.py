@@ -63,8 +63,14 @@ def test_kh_files_and_state():
     dens = text.split("SCALARS density float 1\nLOOKUP_TABLE default\n")[1].split("VECTORS")[0].split()
     rho_file = np.array([float(v) for v in dens]).reshape(2, 24, 24)
     assert np.array_equal(rho_file[0], rho_file[1])
-    # line_extract along y = z = 0.5: cell row 12 of the upper layer
-    assert np.array_equal(line[:, 1], rho_file[1, 12])
+    # line_extract along y = z = 0.5 reads the file's own header: the 6-digit
+    # SPACING of the reference's writer (1/24 -> 0.0416667) puts the probe
+    # points i/24 just below face i, in cell i - 1, as the probe filter does;
+    # y = 0.5 and z = 0.5 fall in cell row 11 (or 12) and layer 0 (or 1)
+    sp = [float(v) for v in text.split("SPACING")[1].split()[:3]]
+    col = [min(max(int(np.floor((i / 24) / sp[0])), 0), 23) for i in range(24)]
+    row = int(np.floor(0.5 / sp[1]))
+    assert np.array_equal(line[:, 1], rho_file[int(np.floor(0.5 / sp[2])), row, col])
     cfg, final, stamps = euler_cfg("eu2d_kh_spweno")
     s = Solver(cfg)
     s.set_state(euler_prim_for(case("euler", "eu2d_kh_spweno")["init"], 2))

@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 import oracle_ctypes as orc
-from cases import case, collapse, euler_cfg, euler_prim_for, fixtures, scalar_cfg
+from cases import (case, collapse, euler_cfg, euler_prim_for, fixtures, mirror_full, ns_box_cfg,
+                   scalar_cfg, tgv_sub_box)
 from tecno import (FP32, FP64, Solver, make_cfg, MLP_BF16, MLP_FP16, MLP_FP32, MLP_FP64,
                    TIME_SCHEME)
 
@@ -178,6 +179,53 @@ def test_time_schemes_mixed_path_close_to_fp64_oracle(ts):
             assert e.max() <= 1e-8, e.max()
     assert np.quantile(e, 0.99) <= 1e-8, np.quantile(e, 0.99)
     assert e.max() <= 2e-5, e.max()
+
+
+@pytest.mark.parametrize("bc,fix_edges", [
+    (["wall", "wall", "neumann", "neumann", "periodic", "periodic"], 0),
+    (["neumann", "neumann", "periodic", "periodic", "wall", "wall"], 0),
+    (["wall", "wall", "wall", "wall", "neumann", "neumann"], 0),
+    ("periodic", 1)])
+def test_ns3d_boundaries_per_step_vs_oracle(bc, fix_edges):
+    """Navier-Stokes with wall / neumann boundaries (an extension: the
+    reference allows periodic only) and with fix_edges: GPU against the
+    oracle, whose extension the CPU tests pin by mirror and uniformity
+    invariants; 3 steps, 1e-12 relative per step."""
+    rng = np.random.default_rng(31)
+    n = (10, 8, 6)
+    cfg = ns_box_cfg(n, (2.0, 1.6, 1.2), bc, fix_edges=fix_edges)
+    h = 0.2
+    z, y, x = np.meshgrid(*[(np.arange(k) + 0.5) * h for k in n[::-1]], indexing="ij")
+    st = np.stack([1.0 + 0.1 * np.sin(2 * x + y), np.sin(3 * y) * np.cos(2 * z), np.cos(2 * x) * np.sin(z),
+                   0.5 * np.sin(x + z), 5.0 + 0.2 * np.cos(x - y)]) + rng.normal(size=(5,) + x.shape) * 0.01
+    init = st.reshape(5, -1)
+    o = orc.OracleSolver(cfg, None, faithful=True)
+    o.set_state(init)
+    s = Solver(cfg)
+    s.set_state(init)
+    for k in range(3):
+        dto = o.step()
+        dtg = s.step()
+        assert abs(dtg - dto) <= 1e-14 * dto
+        assert rel_err(s.get_state(), o.get_state()) <= 1e-12 * (k + 1), (k, rel_err(s.get_state(), o.get_state()))
+
+
+def test_ns3d_walls_are_mirror_planes_gpu():
+    """The GPU run on [0, pi]^3 with walls equals the octant of the periodic
+    [0, 2 pi]^3 run of the mirrored state (every node counted once)."""
+    m = 8
+    sub = tgv_sub_box(m, m, np.random.default_rng(2))
+    full = mirror_full(sub, (0, 1, 2))
+    a = Solver(ns_box_cfg((m, m, m), (math.pi,) * 3, "wall", fix_edges=1))
+    b = Solver(ns_box_cfg((2 * m,) * 3, (2 * math.pi,) * 3, "periodic", fix_edges=1))
+    a.set_state(sub.reshape(5, -1))
+    b.set_state(full.reshape(5, -1))
+    for k in range(4):
+        a.step()
+        b.step()
+        ga = a.get_state().reshape(sub.shape)
+        gb = b.get_state().reshape(full.shape)[:, :m, :m, :m]
+        assert rel_err(ga.reshape(5, -1), gb.reshape(5, -1)) <= 1e-12 * (k + 1)
 
 
 def test_config1_l1_gpu():

@@ -415,6 +415,7 @@ def tgv_bench(tecno, n, t_end, recon, net_file, every=0.25):
     a = s.audit()
     hist = [(0.0, a.kinetic_energy / vol, a.total_entropy / vol)]
     steps, secs = 0, 0.0
+    every = min(every, t_end)
     t_next = every
     while t_next <= t_end + 1e-12:
         _, it0 = s.time()

@@ -183,7 +183,9 @@ struct Tile {
   }
 
   // 1b. two-point fluxes of the distance-2 pairs (PH::PAIRS only): x pairs
-  // row by row, then y pairs column by column, one pair per lane per round
+  // (TX+2 per row) then y pairs (TX per row of TY+2 rows), both row by row,
+  // one pair per lane per round: neighbouring lanes read neighbouring cells,
+  // so the FP64 record reads are free of LDS bank conflicts
   __device__ static void pairs(const Phys<TP>& P, void* lds, int tid) {
     if constexpr (PAIRS) {
       TF* out = (TF*)((unsigned char*)lds + PAIR_OFF);
@@ -196,7 +198,7 @@ struct Tile {
           ax = 0;
         } else {
           const int r = q - SH::NPX;
-          const int fx = r / (TY + 2), py = r - fx * (TY + 2);
+          const int py = r / TX, fx = r - py * TX;
           ca = py * SH::BX + fx + G;
           stride = SH::BX;
           ax = 1;
@@ -214,25 +216,27 @@ struct Tile {
       int f = rd * 256 + tid;
       if (rd * 256 + (tid & ~63) >= NF) break;  // whole wave past the end
       f = f < NF ? f : NF - 1;
-      int base, stride, ax, pq;
+      int base, stride, ax, pq, ps;
       if (f < SH::NFX) {
         const int fy = f / (TX + 1), fx = f - fy * (TX + 1);
         base = (DIMS >= 2 ? (fy + G) * SH::BX : 0) + fx;
         stride = 1;
         ax = 0;
         pq = fy * (TX + 2) + fx;
+        ps = 1;
       } else {
         const int q = f - SH::NFX;
         const int fy = q / TX, fx = q - fy * TX;
         base = fy * SH::BX + fx + G;
         stride = SH::BX;
         ax = 1;
-        pq = SH::NPX + fx * (TY + 2) + fy;
+        pq = SH::NPX + fy * TX + fx;
+        ps = TX;
       }
       unsigned c = 0;
       if constexpr (PAIRS)
         PH::face(P, lds, NB, base, stride, ax, Fr[rd], c,
-                 (const TF*)((const unsigned char*)lds + PAIR_OFF), NP, pq);
+                 (const TF*)((const unsigned char*)lds + PAIR_OFF), NP, pq, ps);
       else
         PH::face(P, lds, NB, base, stride, ax, Fr[rd], c);
       if (rd * 256 + tid < NF) nfb += c;

@@ -504,13 +504,13 @@ struct Euler2Mixed {
     for (int v = 0; v < 4; ++v) out[v * NP + q] = f[v];
   }
 
-  // a face whose pair fluxes are in LDS: pairs q (cells 0,2) and q+1 (1,3)
+  // a face whose pair fluxes are in LDS: pairs q (cells 0,2) and q+ps (1,3)
   __device__ static void face(const Phys<float>& P, const void* ldsv, int NB, int base, int stride,
                               int ax, double F[4], unsigned& nfb, const double* pairs, int NP,
-                              int q) {
+                              int q, int ps) {
     face_core(P, ldsv, NB, base, stride, ax, F, nfb, [&](int pr, double f0[4]) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q + pr];
+      for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q + pr * ps];
     });
   }
 

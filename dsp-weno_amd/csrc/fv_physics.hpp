@@ -282,11 +282,11 @@ struct Euler2Ph {
   // scaled variable and eigenvector entries both change sign on y faces, so
   // the dissipation is unchanged bit for bit.  Sums over the x and y
   // components keep the reference's x-before-y order via `ax`.
-  // pairs: the shared pair fluxes (pairs q and q+1 are this face's F(0,2)
+  // pairs: the shared pair fluxes (pairs q and q+ps are this face's F(0,2)
   // and F(1,3)); nullptr evaluates them here
   __device__ static void face(const Phys<T>& P, const void* ldsv, int NB, int base, int stride,
                               int ax, T F[4], unsigned& nfb, const T* pairs = nullptr,
-                              int NP = 0, int q = 0) {
+                              int NP = 0, int q = 0, int ps = 1) {
     const T* lds = (const T*)ldsv;
     constexpr int S = 2 * G;
     const int rn = ax ? E_UY : E_UX, rt = ax ? E_UX : E_UY;
@@ -356,7 +356,7 @@ struct Euler2Ph {
       for (int v = 0; v < 4; ++v) Fe[v] += f0[v] * T(-1.0 / 6.0);
       if (G == 2 && pairs) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q + 1];
+        for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q + ps];
       } else {
         kepec2(P, rho[L], un[L], ut[L], bk[L], rho[R + 1], un[R + 1], ut[R + 1], bk[R + 1], area,
                f0);

@@ -45,6 +45,24 @@ def test_create_rejects_bad_configs(bad):
     assert tecno.lib().tecno_last_error()
 
 
+@pytest.mark.parametrize("kw,msg", [
+    (dict(bc=["periodic", "wall", "periodic", "periodic", "periodic", "periodic"]), "paired"),
+    (dict(time_scheme=7), "time scheme"),
+    (dict(recon="minmod", precision=tecno.FP32), "FP32 Euler path"),
+    (dict(flux="roe", precision=tecno.FP32), "FP32 Euler path"),
+    (dict(model="ns_es", dims=2), "3D grids"),
+    (dict(model="ns_es", dims=3, n=(8, 8, 1), viscosity=("constant", 1e-3)), "at least 2 cells")])
+def test_create_rejects_with_message(kw, msg):
+    """The messages name the rule that failed (periodic pairing as
+    parameter.cc:570-581, the supported precision combinations, ...)."""
+    args = dict(n=(16, 16, 2), dims=2)
+    args.update(kw)
+    cfg = tecno.make_cfg(**args)
+    h = C.c_void_p()
+    assert tecno.lib().tecno_create(C.byref(cfg), C.byref(h)) == -1
+    assert msg in tecno.lib().tecno_last_error().decode(), tecno.lib().tecno_last_error()
+
+
 def test_cfg_struct_layout_matches_header():
     """The ctypes mirror of tecno_cfg has the size of the C struct
     (sizeof(tecno_cfg) == 272 with gcc/hipcc on x86-64)."""

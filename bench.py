@@ -483,8 +483,9 @@ def main():
         print("copied 1073741824 bytes")
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU decomposition is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one GPU per run; --gpus {args.gpus} is not supported (a run holds one "
+              f"GPU's share of the grid; slab decomposition with halo exchange is not built)",
+              file=sys.stderr)
         sys.exit(2)
     import tecno
 

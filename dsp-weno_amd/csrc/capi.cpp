@@ -291,7 +291,7 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
     return fail(TECNO_EINVAL, "Euler: 2D or 3D grids");
   if (g.system == TECNO_SYS_EULER && g.dims == 3) {
     if (g.precision != TECNO_FP64)
-      return fail(TECNO_EINVAL, "3D Euler / Navier-Stokes runs in FP64 in this build");
+      return fail(TECNO_EINVAL, "3D Euler / Navier-Stokes runs in FP64 only");
     if (g.recon == TECNO_REC_ENO && g.eno_level == 3)
       return fail(TECNO_EINVAL, "3D: reconstructions with 2 ghost cells (eno 3 is 2D only)");
   }
@@ -308,7 +308,7 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
       return fail(TECNO_EINVAL, "unknown viscosity model");
   }
   if (g.system == TECNO_SYS_SCALAR && g.dims == 3)
-    return fail(TECNO_EINVAL, "scalar: 3D grids are not supported by this build");
+    return fail(TECNO_EINVAL, "scalar: 3D grids are not supported (1D and 2D)");
   if (g.system == TECNO_SYS_SCALAR && (g.flux < 0 || g.flux > 7))
     return fail(TECNO_EINVAL, "unknown scalar flux");
   if (g.system == TECNO_SYS_EULER && !((g.flux >= 100 && g.flux <= 106) || (g.flux >= 120 && g.flux <= 128)))
@@ -862,7 +862,7 @@ int tecno_mlp_forward_batch(tecno_ctx* c, const double* u4, int64_t n, double* l
 int tecno_flux_batch(tecno_ctx* c, const double* P, const double* V, int64_t n, int axis,
                      double* F) {
   if (!c || !P || !F) return fail(TECNO_EINVAL, "null argument");
-  if (axis < 0 || axis > 1) return fail(TECNO_EINVAL, "axis must be 0 or 1 for this build");
+  if (axis < 0 || axis > 1) return fail(TECNO_EINVAL, "axis must be 0 or 1 (2D faces)");
   if (n <= 0) return 0;
   if (int rc = need_net(c)) return rc;
   const int ng = c->G;

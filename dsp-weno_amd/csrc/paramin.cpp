@@ -397,7 +397,7 @@ void read_numeric(Reader& r, Param& p) {
     c.recon = TECNO_REC_ENO;
     c.eno_level = r.integer();
   } else
-    throw InputError("reconstruct " + rec + " is not provided by this build");
+    throw InputError("unknown reconstruct " + rec);
   r.entry("net_file");
   p.net_file = r.token();
   r.end_section();
@@ -480,7 +480,7 @@ void read_material(Reader& r, Param& p) {
     c.flux = -1;
     for (auto& nm : names)
       if (f == nm.first) c.flux = nm.second;
-    if (c.flux < 0) throw InputError("flux " + f + " is not provided by this build");
+    if (c.flux < 0) throw InputError("unknown flux " + f);
     r.entry("EC1_fix");
     const std::string e = r.token();
     if (e == "yes") c.ec1_fix = 1;

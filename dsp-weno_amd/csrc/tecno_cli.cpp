@@ -282,7 +282,7 @@ int main(int argc, char** argv) {
       r.dz = (p.zmax - p.zmin) / r.Nz;
       tecno_cfg cfg = p.base;
       r.dims = p.scalar ? p.pdim : std::max(p.pdim, 2);
-      if (p.scalar && r.dims == 3) throw std::runtime_error("scalar: 3D problems are not provided by this build");
+      if (p.scalar && r.dims == 3) throw std::runtime_error("scalar: 3D problems are not supported (1D and 2D)");
       cfg.dims = r.dims;
       cfg.n[0] = r.Nx;
       cfg.n[1] = r.Ny;

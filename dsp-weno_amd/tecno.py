@@ -150,7 +150,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+            raise ImportError(f"{LIB_PATH} is missing: build it with make -C dsp-weno_amd")
         L = C.CDLL(LIB_PATH)
         P, D, I64, I32 = C.c_void_p, C.c_double, C.c_int64, C.c_int
         dp = C.POINTER(C.c_double)

@@ -216,7 +216,7 @@ def test_euler_2d_reduced_dims_bitwise(key):
 
 def test_config1_l1_error():
     """Config 1 (linadv sin^4 x, lin_tecno4 + sp_weno, Nx=400, t=0.5): the
-    reference's published-in-survey L1 error 1.75268e-05 (65 steps)."""
+    reference run's own L1 error 1.75268e-05 (65 steps)."""
     d = case("scalar", "config1_spweno")
     cfg, final, stamps = scalar_cfg("config1_spweno")
     s = orc.OracleSolver(cfg, None, faithful=False)

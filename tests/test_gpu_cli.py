@@ -1,5 +1,6 @@
 """`tecno -i param.in` end to end on the GPU: the reference's OOC tables
 (config 1 figures from the reference run) and the solution / globals files."""
+import math
 import os
 import re
 import subprocess
@@ -10,7 +11,7 @@ import pytest
 
 import line_extract as LE
 from cases import euler_cfg, euler_prim_for, case
-from tecno import Solver
+from tecno import TIME_SCHEME, Solver, make_cfg
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -79,3 +80,31 @@ def test_kh_files_and_state():
     rho = s.get_state()[0].reshape(24, 24)
     assert np.allclose(rho_file[0], rho, rtol=1e-5, atol=0)
     assert float(glob[0].split()[0]) == 0.0
+
+
+@pytest.mark.parametrize("ts,rec", [("rk4", "minmod"), ("jameson_rk4", "second"), ("rk1", "minabs"),
+                                    ("ssprk3", "first")])
+def test_time_schemes_and_reconstructions_end_to_end(ts, rec):
+    """`tecno -i param.in` with the other time_scheme / reconstruct names of
+    parameter.cc (config 1's file at Nx = 100): the last solution file holds
+    the state the library reaches for the same configuration (6 digits, as
+    the file is written)."""
+    text = open(os.path.join(CASES, "config1_linadv_spweno.param.in")).read()
+    text = text.replace("Nx_cell { 400 }", "Nx_cell { 100 }")
+    text = re.sub(r"time_scheme\s+\S+", f"time_scheme  {ts}", text)
+    text = re.sub(r"reconstruct\s+\S+", f"reconstruct  {rec}", text)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "param.in")
+        open(p, "w").write(text)
+        run_cli(p, d)
+        out = os.path.join(d, f"SOLN_FILES_{rec}")
+        last = sorted(f for f in os.listdir(out) if f.startswith("sol"))[-1]
+        _, _, _, fields = LE.read_vtk(os.path.join(out, last))
+    u_file = fields["solution"].reshape(2, 2, 100)[0, 0]
+    cfg = make_cfg(system="scalar", n=(100, 2, 2), dims=1, lo=(-math.pi, 0, 0), hi=(math.pi, 1, 1),
+                   recon=rec, flux="lin_tecno4", time_scheme=TIME_SCHEME[ts])
+    s = Solver(cfg)
+    x = -math.pi + (np.arange(100) + 0.5) * 2 * math.pi / 100
+    s.set_state(np.sin(x) ** 4)
+    s.advance(0.5, 5)
+    assert np.allclose(u_file, s.get_state()[0], rtol=2e-5, atol=2e-6)

@@ -40,7 +40,7 @@ def scalar_cfg(key, **kw):
     if key.startswith("sc1d_ts_"):
         return make_cfg(n=(64, 2, 2), recon="sp_weno", flux="lin_tecno4",
                         time_scheme=TIME_SCHEME[key[8:]], **SC1D, **kw), 0.5, 5
-    if key == "config1_spweno":
+    if key == "linadv400_spweno":
         return make_cfg(n=(400, 2, 2), recon="sp_weno", flux="lin_tecno4", **SC1D, **kw), 0.5, 5
     if key.startswith("sc1d_"):
         flux = key[5:]

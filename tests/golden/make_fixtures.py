@@ -223,14 +223,14 @@ def scalar_cases(tmp):
         ns = len(t)
         fx[f"sc1d_{tag}"] = dict(init=init, times=t, dts=dt, states=st[[0, 4, ns - 1]],
                                  steps=np.array([1, 5, ns]))
-    # config 1 itself: Nx=400 to t=0.5 (65 steps), final state
+    # linear advection of sin^4 x at Nx=400 to t=0.5 (65 steps), final state
     text = param_in(n=(400, 2, 2), recon="sp_weno", flux="lin_tecno4", **base)
-    with open(os.path.join(CASES, "config1_linadv_spweno.param.in"), "w") as f:
+    with open(os.path.join(CASES, "linadv400_spweno.param.in"), "w") as f:
         f.write(param_in(n=(400, 2, 2), recon="sp_weno", flux="lin_tecno4", system="scalar",
                          lo=(-np.pi, 0.0, 0.0), hi=(np.pi, 1.0, 1.0), final_time=0.5,
                          ic="pow(sin(x),4)", pdim=1, out=".", exact="pow(sin(x-t),4)"))
     init, t, dt, st = harness_run(REFH_SCA, text, 1000, tmp)
-    fx["config1_spweno"] = dict(init=init, times=t, dts=dt, states=st[[0, -1]], steps=np.array([1, len(t)]))
+    fx["linadv400_spweno"] = dict(init=init, times=t, dts=dt, states=st[[0, -1]], steps=np.array([1, len(t)]))
     # other fluxes in 1D (lin_ec4 / lin_tecno / bur_*)
     for flux, model, ic in [("lin_tecno", None, "pow(sin(x),4)"), ("lin_ec4", None, "pow(sin(x),4)"),
                             ("bur_tecno4", "burgers", "sin(x)"), ("bur_tecno", "burgers", "sin(x)"),

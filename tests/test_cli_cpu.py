@@ -33,7 +33,7 @@ def dry(path):
 
 
 @pytest.mark.parametrize("case,fx,key", [
-    ("config1_linadv_spweno.param.in", "scalar", "config1_spweno"),
+    ("linadv400_spweno.param.in", "scalar", "linadv400_spweno"),
     ("kh_spweno_24.param.in", "euler", "eu2d_kh_spweno"),
     ("kh_dsp_24.param.in", "euler", "eu2d_kh_dsp")])
 def test_initial_condition_matches_reference(case, fx, key):
@@ -55,7 +55,7 @@ def test_parsed_fields():
     assert out["reconstruction"] == ["sp_weno_dl"] and out["t_stamps"] == ["5"]
     out, _ = dry(os.path.join(CASES, "vortex_spweno_ooc.param.in"))
     assert out["levels"] == ["2"] and out["ooc"] == ["1"] and out["pdim"] == ["2"]
-    out, _ = dry(os.path.join(CASES, "config1_linadv_spweno.param.in"))
+    out, _ = dry(os.path.join(CASES, "linadv400_spweno.param.in"))
     assert out["system"] == ["scalar"] and out["flux"] == ["3"] and out["exact"] == ["1"]
 
 

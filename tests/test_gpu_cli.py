@@ -1,5 +1,5 @@
 """`tecno -i param.in` end to end on the GPU: the reference's OOC tables
-(config 1 figures from the reference run) and the solution / globals files."""
+(the linear-advection sin^4 figures of the reference run) and the solution / globals files."""
 import math
 import os
 import re
@@ -26,10 +26,10 @@ def run_cli(param, cwd):
     return r
 
 
-def test_config1_ooc_table():
+def test_linadv_sin4_ooc_table():
     """lin_tecno4 + sp_weno, sin^4 on [-pi, pi], Nx = 100..800, t = 0.5: the
     reference reports L1 = 1.75268e-05 at Nx = 400 and orders 3.17, 3.21, 3.25"""
-    text = open(os.path.join(CASES, "config1_linadv_spweno.param.in")).read()
+    text = open(os.path.join(CASES, "linadv400_spweno.param.in")).read()
     text = text.replace("Nx_cell { 400 }", "Nx_cell { 100 200 400 800 }")
     text = text.replace("Ny_cell { 2 }", "Ny_cell { 2 2 2 2 }").replace("Nz_cell { 2 }", "Nz_cell { 2 2 2 2 }")
     text = text.replace("OOC_study   no", "OOC_study   yes")
@@ -86,10 +86,10 @@ def test_kh_files_and_state():
                                     ("ssprk3", "first")])
 def test_time_schemes_and_reconstructions_end_to_end(ts, rec):
     """`tecno -i param.in` with the other time_scheme / reconstruct names of
-    parameter.cc (config 1's file at Nx = 100): the last solution file holds
+    parameter.cc (the sin^4 linear-advection file at Nx = 100): the last solution file holds
     the state the library reaches for the same configuration (6 digits, as
     the file is written)."""
-    text = open(os.path.join(CASES, "config1_linadv_spweno.param.in")).read()
+    text = open(os.path.join(CASES, "linadv400_spweno.param.in")).read()
     text = text.replace("Nx_cell { 400 }", "Nx_cell { 100 }")
     text = re.sub(r"time_scheme\s+\S+", f"time_scheme  {ts}", text)
     text = re.sub(r"reconstruct\s+\S+", f"reconstruct  {rec}", text)

@@ -228,11 +228,11 @@ def test_ns3d_walls_are_mirror_planes_gpu():
         assert rel_err(ga.reshape(5, -1), gb.reshape(5, -1)) <= 1e-12 * (k + 1)
 
 
-def test_config1_l1_gpu():
+def test_linadv_sin4_l1_gpu():
     """Config 1: lin_tecno4 + sp_weno, Nx = 400, t = 0.5 -> L1 = 1.75268e-05
     (the reference's own figure), 65 steps."""
-    d = case("scalar", "config1_spweno")
-    cfg, final, stamps = scalar_cfg("config1_spweno")
+    d = case("scalar", "linadv400_spweno")
+    cfg, final, stamps = scalar_cfg("linadv400_spweno")
     s = Solver(cfg)
     s.set_state(collapse(d["init"], 1).reshape(-1))
     it, t = s.advance(final, stamps)

@@ -214,11 +214,11 @@ def test_euler_2d_reduced_dims_bitwise(key):
         np.testing.assert_array_equal(out[int(k)], ref, err_msg=f"step {k}")
 
 
-def test_config1_l1_error():
+def test_linadv_sin4_l1_error():
     """Config 1 (linadv sin^4 x, lin_tecno4 + sp_weno, Nx=400, t=0.5): the
     reference run's own L1 error 1.75268e-05 (65 steps)."""
-    d = case("scalar", "config1_spweno")
-    cfg, final, stamps = scalar_cfg("config1_spweno")
+    d = case("scalar", "linadv400_spweno")
+    cfg, final, stamps = scalar_cfg("linadv400_spweno")
     s = orc.OracleSolver(cfg, None, faithful=False)
     s.set_state(collapse(d["init"], 1))
     it, t = s.advance(final, stamps)

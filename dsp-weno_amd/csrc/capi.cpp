@@ -27,6 +27,11 @@ using namespace tecno;
 
 static thread_local std::string g_err;
 
+static int fail(int code, const std::string& msg);
+namespace tecno {
+// for the host-only entry points in other files of the library
+int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace tecno
 static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;

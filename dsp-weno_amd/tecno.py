@@ -155,6 +155,8 @@ def lib():
         P, D, I64, I32 = C.c_void_p, C.c_double, C.c_int64, C.c_int
         dp = C.POINTER(C.c_double)
         L.tecno_create.argtypes = [C.POINTER(Cfg), C.POINTER(P)]
+        L.tecno_read_param.argtypes = [C.c_char_p, I32, C.POINTER(Cfg), C.POINTER(C.c_int), dp,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, I32]
         L.tecno_destroy.argtypes = [P]
         L.tecno_destroy.restype = None
         L.tecno_load_net.argtypes = [P, C.c_char_p]
@@ -377,6 +379,20 @@ class Solver:
         n = C.c_int64()
         _check(lib().tecno_timing_get(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def read_param(path, level=0):
+    """Parameter::read of a param.in file through the library
+    (tecno_read_param): the tecno_cfg of mesh level `level` and a dict of
+    levels, final_time, time_stamps, max_iter, net_file."""
+    cfg = Cfg()
+    nl, ts, mi = C.c_int(), C.c_int(), C.c_int()
+    ft = C.c_double()
+    buf = C.create_string_buffer(4096)
+    _check(lib().tecno_read_param(str(path).encode(), level, C.byref(cfg), C.byref(nl), C.byref(ft),
+                                  C.byref(ts), C.byref(mi), buf, len(buf)))
+    return cfg, dict(levels=nl.value, final_time=ft.value, time_stamps=ts.value, max_iter=mi.value,
+                     net_file=buf.value.decode())
 
 
 def make_train_rows(u4, target):

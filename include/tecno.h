@@ -135,6 +135,19 @@ typedef struct tecno_ctx tecno_ctx;
 /* Fill *cfg with defaults (2D Euler, kepes_tecno4_roe, sp_weno, fp64). */
 void tecno_cfg_default(tecno_cfg* cfg);
 
+/* Parameter::read (enflo/src/parameter.cc:16-29 with its section readers
+ * 35-700; enflo_scalar/src_scalar/parameter.cc for `model linadv|burgers`
+ * files): parse a param.in file into the tecno_cfg of mesh level `level`
+ * (dims from problem_dim, at least 2 for systems; n, lo, hi of that level;
+ * precision FP64 and the fix flags 0 - set them afterwards).  Optional
+ * outputs (NULL to skip): the number of mesh levels, final_time,
+ * time_stamps, max_iter, and net_file copied into a buffer of net_len
+ * bytes.  A malformed file or an unknown name returns TECNO_EINVAL with the
+ * reader's message. */
+int tecno_read_param(const char* path, int level, tecno_cfg* cfg, int* n_levels,
+                     double* final_time, int* time_stamps, int* max_iter, char* net_file,
+                     int net_len);
+
 /* Create a context; allocates device state.  Replaces FV::FV + FV::initialize
  * (enflo/src/fv.h:24-67, fv.cc:18-93). */
 int tecno_create(const tecno_cfg* cfg, tecno_ctx** out);

@@ -173,3 +173,39 @@ def test_missing_file():
     r = subprocess.run([CLI, "-i", "/nonexistent/param.in", "--dry-run"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode != 0 and "cannot open" in r.stderr
+
+
+@pytest.mark.parametrize("case", sorted(os.listdir(CASES)))
+def test_library_param_reader_matches_command_line(case):
+    """tecno_read_param (the library's Parameter::read) gives the fields the
+    command line parses from the same file."""
+    import tecno
+    out, _ = dry(os.path.join(CASES, case))
+    cfg, info = tecno.read_param(os.path.join(CASES, case))
+    assert int(out["flux"][0]) == cfg.flux
+    assert [int(v) for v in out["recon"]] == [cfg.recon, cfg.spweno_corr, cfg.eno_level]
+    assert int(out["time_scheme"][0]) == cfg.time_scheme
+    assert [int(v) for v in out["bc"]] == list(cfg.bc)
+    assert float(out["cfl"][0]) == cfg.cfl
+    assert int(out["levels"][0]) == info["levels"]
+    assert float(out["final_time"][0]) == info["final_time"]
+    assert int(out["t_stamps"][0]) == info["time_stamps"]
+    assert (cfg.system == tecno.SYS_SCALAR) == (out["system"] == ["scalar"])
+    assert cfg.dims == (int(out["pdim"][0]) if out["system"] == ["scalar"] else max(2, int(out["pdim"][0])))
+
+
+def test_library_param_reader_errors(tmp_path):
+    """a missing file, an unknown name and a null argument fail with the
+    reader's message"""
+    import tecno
+    with pytest.raises(tecno.TecnoError, match="cannot open"):
+        tecno.read_param(tmp_path / "missing.param.in")
+    p = tmp_path / "bad.param.in"
+    p.write_text(SCALAR.format(expr="x", xmin=0.0, xmax=1.0, ts="rk5", rec="sp_weno",
+                               flux="lin_tecno4", bc="periodic"))
+    with pytest.raises(tecno.TecnoError, match="time_scheme rk5"):
+        tecno.read_param(p)
+    with pytest.raises(tecno.TecnoError, match="mesh level"):
+        tecno.read_param(os.path.join(CASES, "kh_spweno_24.param.in"), level=1)
+    assert tecno.lib().tecno_read_param(str(p).encode(), 0, None, None, None, None, None, None, 0) == -1
+    assert b"null argument" in tecno.lib().tecno_last_error()

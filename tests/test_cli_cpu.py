@@ -2,6 +2,7 @@
 param.in reader and the expression evaluator against the reference's own
 initial states (the golden fixtures hold them), the error paths, and
 operator semantics."""
+import glob
 import os
 import subprocess
 import tempfile
@@ -209,3 +210,19 @@ def test_library_param_reader_errors(tmp_path):
         tecno.read_param(os.path.join(CASES, "kh_spweno_24.param.in"), level=1)
     assert tecno.lib().tecno_read_param(str(p).encode(), 0, None, None, None, None, None, None, 0) == -1
     assert b"null argument" in tecno.lib().tecno_last_error()
+
+
+REF_EXAMPLES = sorted(glob.glob("/root/reference/*/examples/**/param.in", recursive=True))
+
+
+@pytest.mark.skipif(not REF_EXAMPLES, reason="reference examples not present on this machine")
+@pytest.mark.parametrize("path", REF_EXAMPLES or ["none"])
+def test_reference_examples_parse(path):
+    """every param.in the reference ships under examples/ (enflo: 1D/2D Euler;
+    enflo_scalar: Burgers, linear advection) is read by the library's reader
+    and by the command line's dry run"""
+    import tecno
+    cfg, info = tecno.read_param(path)
+    assert info["levels"] >= 1 and info["final_time"] > 0 and info["net_file"]
+    out, ic = dry(path)
+    assert int(out["flux"][0]) == cfg.flux and len(ic) in (1, 5)

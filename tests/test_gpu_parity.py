@@ -228,6 +228,38 @@ def test_ns3d_walls_are_mirror_planes_gpu():
         assert rel_err(ga.reshape(5, -1), gb.reshape(5, -1)) <= 1e-12 * (k + 1)
 
 
+@pytest.mark.parametrize("rec", ["sp_weno", "sp_weno_dl"])
+def test_euler_1d_shock_tube_vs_oracle(rec):
+    """The reference's 1D Euler examples run as Nx x 2 x 2 grids (problem_dim
+    1): Lax's shock tube (rho, u, p = 0.445, 0.698, 3.528 | 0.5, 0, 0.571 on
+    [-5, 5], Nx = 200, neumann in x, periodic in y), CFL 0.4 - 20 steps of
+    the GPU FP64 path within 1e-12 relative per step of the oracle."""
+    nx = 200
+    cfg = make_cfg(n=(nx, 2, 2), dims=2, lo=(-5, 0, 0), hi=(5, 1, 1), recon=rec, cfl=0.4,
+                   bc=["neumann", "neumann", "periodic", "periodic", "periodic", "periodic"])
+    x = -5 + (np.arange(nx) + 0.5) * 10 / nx
+    left = x < 0
+    rho = np.where(left, 0.445, 0.5)
+    u = np.where(left, 0.698, 0.0)
+    p = np.where(left, 3.528, 0.571)
+    prim = np.stack([np.tile(rho, 2), np.tile(u, 2), np.zeros(2 * nx), np.tile(p, 2)])
+    net = orc.load_net() if rec == "sp_weno_dl" else None
+    o = orc.OracleSolver(cfg, net, faithful=False)
+    full = np.zeros((5, 2 * nx))
+    full[[0, 1, 2, 4]] = prim
+    o.set_state(full)
+    s = solver_for(cfg, rec == "sp_weno_dl")
+    s.set_state(prim)
+    for k in range(20):
+        dto = o.step()
+        dtg = s.step()
+        assert abs(dtg - dto) <= 1e-14 * dto
+        ref = o.get_state()[[0, 1, 2, 4]]
+        assert rel_err(s.get_state(), ref) <= 1e-12 * (k + 1), (k, rel_err(s.get_state(), ref))
+    st = s.get_state()
+    assert st[0].min() > 0 and st[3].min() > 0 and np.array_equal(st[:, :nx], st[:, nx:])
+
+
 def test_linadv_sin4_l1_gpu():
     """Config 1: lin_tecno4 + sp_weno, Nx = 400, t = 0.5 -> L1 = 1.75268e-05
     (the reference's own figure), 65 steps."""

@@ -31,6 +31,8 @@ METRIC = "cell-updates/s, 2D Euler 2048² DSP-WENO mixed vs FP64 SP-WENO at equa
 NET_FILE = os.path.join(ROOT, "tests", "golden", "dsp_weno_net.txt")
 HBM_PEAK_GBS = 8000.0
 NVAR = 4
+# untimed steps from the KH initial state before any path is timed (t ~ 0.03)
+SPINUP_STEPS = 300
 
 
 def kh_state(n, seed=2024):
@@ -45,13 +47,21 @@ def kh_state(n, seed=2024):
     return np.stack([rho, u, v, p]).reshape(4, -1), a
 
 
-def make_solver(tecno, n, precision, recon, mlp_dtype):
+XAVIER_SEED = 6008  # the trainer's default seed (config.py)
+
+
+def make_solver(tecno, n, precision, recon, mlp_dtype, weights="shipped"):
+    """weights: "shipped" (the net_file) or "xavier" (seeded Xavier-uniform
+    weights of the same 5-5-5-5-5 architecture, tecno_init_net)"""
     cfg = tecno.make_cfg(n=(n, n, 2), dims=2, lo=(0, 0, 0), hi=(1, 1, 1), recon=recon,
                          flux="kepes_tecno4_roe", bc="periodic", cfl=0.5, precision=precision,
                          mlp_dtype=mlp_dtype, sign_fallback=1)
     s = tecno.Solver(cfg)
     if recon == "sp_weno_dl":
-        s.load_net(NET_FILE)
+        if weights == "xavier":
+            s.init_net([5, 5, 5, 5, 5], XAVIER_SEED)
+        else:
+            s.load_net(NET_FILE)
     return s
 
 
@@ -165,22 +175,27 @@ output
 """
 
 
-def cpu_baseline(n_sample=512, steps=2):
+def _kh_param_file(tmp, n_sample):
+    a = np.random.default_rng(2024).normal(size=8)
+    consts = "\n".join(f"   a{k}   {float(a[k])!r}" for k in range(8))
+    pert = " + ".join(f"a{k}*sin(2*pi*{k + 1}*x)" for k in range(8))
+    p = os.path.join(tmp, "param.in")
+    with open(p, "w") as f:
+        f.write(KH_PARAM.format(n=n_sample, rec="sp_weno_dl", net=NET_FILE, consts=consts,
+                                pert=pert, out=tmp))
+    return p
+
+
+def cpu_baseline(n_sample=1024, steps=1):
     """The reference solver (built from its sources into oracle/_ref by
     oracle/Makefile) on one host core, on a bounded sample of the workload:
     the KH problem at n_sample^2 (the reference's Nz = 2 layout), `steps`
-    steps of sp_weno_dl.  Falls back to the oracle's C port when the
-    reference build is absent."""
+    steps of sp_weno_dl (about 15 s).  Falls back to the oracle's C port when
+    the reference build is absent."""
     refh = os.path.join(ROOT, "oracle", "_ref", "refh_sys")
-    a = np.random.default_rng(2024).normal(size=8)
     if os.path.exists(refh):
         with tempfile.TemporaryDirectory() as tmp:
-            consts = "\n".join(f"   a{k}   {float(a[k])!r}" for k in range(8))
-            pert = " + ".join(f"a{k}*sin(2*pi*{k + 1}*x)" for k in range(8))
-            p = os.path.join(tmp, "param.in")
-            with open(p, "w") as f:
-                f.write(KH_PARAM.format(n=n_sample, rec="sp_weno_dl", net=NET_FILE, consts=consts,
-                                        pert=pert, out=tmp))
+            p = _kh_param_file(tmp, n_sample)
             r = subprocess.run(["taskset", "-c", "0", refh, "time", p, str(steps)], capture_output=True,
                                text=True, timeout=600)
             if r.returncode == 0:
@@ -191,12 +206,17 @@ def cpu_baseline(n_sample=512, steps=2):
                         "sample": f"KH {n_sample}x{n_sample} (x2 z-layers as the reference stores 2D), "
                                   f"{steps} SSPRK3 step(s), sp_weno_dl FP64, reference solver built from "
                                   f"source, 1 host core, {sec:.1f} s"}
-    # fallback: the oracle (C restatement), faithful mode, one thread
+    return _oracle_baseline(n_sample, steps, 1)
+
+
+def _oracle_baseline(n_sample, steps, threads):
+    """The oracle (the C restatement of the reference, its arithmetic and Nz = 2
+    layout) on `threads` host threads (OpenMP over face rows)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes as orc
     import tecno
     cfg = tecno.make_cfg(n=(n_sample, n_sample, 2), dims=2, recon="sp_weno_dl", bc="periodic")
-    s = orc.OracleSolver(cfg, orc.load_net(NET_FILE), faithful=True, threads=1)
+    s = orc.OracleSolver(cfg, orc.load_net(NET_FILE), faithful=True, threads=threads)
     prim, _ = kh_state(n_sample)
     full = np.zeros((5, 2, n_sample * n_sample))
     full[[0, 1, 2, 4]] = prim[:, None, :]
@@ -205,9 +225,36 @@ def cpu_baseline(n_sample=512, steps=2):
     for _ in range(steps):
         s.step(1.0)
     sec = time.perf_counter() - t0
-    return {"value": n_sample * n_sample * steps / sec, "unit": "cell-updates/s", "cores": 1,
+    return {"value": n_sample * n_sample * steps / sec, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"KH {n_sample}x{n_sample}x2, {steps} step(s), sp_weno_dl, oracle C port, 1 core"}
+            "sample": f"KH {n_sample}x{n_sample}x2, {steps} SSPRK3 step(s), sp_weno_dl FP64, oracle "
+                      f"C port of the reference, {threads} host thread(s), {sec:.1f} s"}
+
+
+def cpu_baseline_all_cores(n_sample=1024, steps=2):
+    """The same workload sample on all host cores available to this process
+    (at most 16): the oracle with OpenMP threads, since the reference solver
+    is serial."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return _oracle_baseline(n_sample, steps, max(1, min(16, cores)))
+
+
+def state_deviation(a, b, steps):
+    """Deviation of state a from state b (nv x N primitive): per variable the
+    largest |a - b| relative to max|b|, and the fraction of cells whose largest
+    relative deviation over the variables is below 1e-8, 1e-6 and 1e-4."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    scale = np.abs(b).max(axis=1, keepdims=True)
+    d = np.abs(a - b) / scale
+    per_cell = d.max(axis=0)
+    return {"steps": int(steps), "max_rel_per_var": [float(x) for x in d.max(axis=1)],
+            "vars": ["rho", "u", "v", "p"],
+            "frac_cells_within": {t: float(np.mean(per_cell <= float(t)))
+                                  for t in ("1e-8", "1e-6", "1e-4")}}
 
 
 VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: CUs x SIMDs x clock / 4 cycles
@@ -364,7 +411,7 @@ def l1_gate(tecno, ns=(128, 256, 512), t_end=1.0, net_file=None):
         for n in ns:
             cfg = tecno.make_cfg(n=(n, n, 2), dims=2, lo=(-5, -5, 0), hi=(5, 5, 1), recon=rec,
                                  flux="kepes_tecno4_roe", bc="periodic", cfl=0.5, precision=prec,
-                                 mlp_dtype=mlp, sign_fallback=int(prec == tecno.FP32))
+                                 mlp_dtype=mlp, sign_fallback=1)
             s = tecno.Solver(cfg)
             if rec == "sp_weno_dl":
                 s.load_net(net_file or NET_FILE)
@@ -379,6 +426,15 @@ def l1_gate(tecno, ns=(128, 256, 512), t_end=1.0, net_file=None):
     h, d, b = out["headline"]["l1"], out["fp64_dsp"]["l1"], out["fp64_spweno"]["l1"]
     out["l1_ratio_vs_fp64_dsp"] = [round(x / y, 4) for x, y in zip(h, d)]
     out["l1_ratio_vs_fp64_spweno"] = [round(x / y, 4) for x, y in zip(h, b)]
+
+    def same(a, ref):
+        # L1 within 2% on every mesh and the convergence orders within 0.05
+        oa, orf = out[a]["order"], out[ref]["order"]
+        return bool(all(abs(x / y - 1) <= 0.02 for x, y in zip(out[a]["l1"], out[ref]["l1"])) and
+                    all(abs(x - y) <= 0.05 for x, y in zip(oa, orf)))
+    out["pass_vs_fp64_spweno"] = same("headline", "fp64_spweno")
+    out["pass_vs_fp64_dsp"] = same("headline", "fp64_dsp")
+    out["rule"] = "L1 within 2% on every mesh and convergence orders within 0.05"
     return out
 
 
@@ -457,11 +513,12 @@ def load_traffic(variant):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="", help="headline | fp64 | fp64dsp (profiling runs)")
+    ap.add_argument("--only", default="",
+                    help="headline | fp64 | fp64dsp (profiling runs) | matrix (network variants)")
     ap.add_argument("--ensemble", action="store_true",
                     help="run the 48-problem scalar ensemble (batched, hipGraph) and exit")
     ap.add_argument("--no-graph", action="store_true", help="ensemble without the hipGraph")
@@ -519,36 +576,46 @@ def main():
     n = args.n
     head_prec, head_rec, head_mlp = tecno.FP32, "sp_weno_dl", tecno.MLP_BF16
     res = {}
-    # both paths are timed from the same developed state: the headline solver
-    # advances the KH initial state by `warmup` steps (untimed), its state is
-    # copied into the FP64 solver
+    # Every path is timed from the same developed state: the headline solver
+    # advances the KH initial state by SPINUP_STEPS untimed steps (t ~ 0.03,
+    # the shear layer has rolled up into resolved eddies), and each path then
+    # starts from a copy of that state, takes `warmup` untimed steps and
+    # `steps` timed ones.  The profiles under profiles/ are taken on the same
+    # state (same spin-up), so their per-launch counters apply to this run.
     sh = make_solver(tecno, n, head_prec, head_rec, head_mlp)
     prim, _ = kh_state(n)
     sh.set_state(prim)
-    sh.run_steps(args.warmup, final_time=1.0, t_stamps=1)
+    sh.run_steps(SPINUP_STEPS, final_time=1.0, t_stamps=1)
     sh.sync()
     spun = sh.get_state()
     t_spun, _ = sh.time()
-    if args.only in ("", "headline"):
-        res["headline"] = time_steps(sh, n, args.steps)
-        print(f"# headline (mixed FP64/FP32, sp_weno_dl, bf16 MFMA network) from t={t_spun:.4f}: "
-              f"{res['headline']}", file=sys.stderr, flush=True)
     sh.close()
-    if args.only in ("", "fp64dsp"):
-        sd = make_solver(tecno, n, tecno.FP64, "sp_weno_dl", tecno.MLP_FP64)
-        sd.set_state(spun)
-        sd.run_steps(1, final_time=1.0, t_stamps=1)
-        res["fp64_dsp"] = time_steps(sd, n, args.steps)
-        sd.close()
-        print(f"# FP64 sp_weno_dl: {res['fp64_dsp']}", file=sys.stderr, flush=True)
-    if args.only in ("", "fp64"):
-        sb = make_solver(tecno, n, tecno.FP64, "sp_weno", tecno.MLP_FP64)
-        sb.set_state(spun)
-        sb.run_steps(1, final_time=1.0, t_stamps=1)
-        res["fp64_spweno"] = time_steps(sb, n, args.steps)
-        sb.close()
-        print(f"# FP64 sp_weno: {res['fp64_spweno']}", file=sys.stderr, flush=True)
-    if args.only:
+    finals = {}
+    # name: (precision, reconstruction, network arithmetic, weights)
+    paths = {"headline": (head_prec, head_rec, head_mlp, "shipped"),
+             "fp64_dsp": (tecno.FP64, "sp_weno_dl", tecno.MLP_FP64, "shipped"),
+             "fp64_spweno": (tecno.FP64, "sp_weno", tecno.MLP_FP64, "shipped"),
+             # config 3's comparison matrix: the network as an fp16 matrix-core
+             # GEMM and as FP32 SIMT, and seeded Xavier weights
+             "mixed_fp16_mfma": (tecno.FP32, "sp_weno_dl", tecno.MLP_FP16, "shipped"),
+             "mixed_fp32_simt": (tecno.FP32, "sp_weno_dl", tecno.MLP_FP32, "shipped"),
+             "mixed_bf16_mfma_xavier": (tecno.FP32, "sp_weno_dl", tecno.MLP_BF16, "xavier"),
+             "fp64_dsp_xavier": (tecno.FP64, "sp_weno_dl", tecno.MLP_FP64, "xavier")}
+    sel = {"": tuple(paths), "headline": ("headline",), "fp64": ("fp64_spweno",),
+           "fp64dsp": ("fp64_dsp",), "matrix": tuple(paths)}[args.only]
+    for name in sel:
+        prec, rec, mlp, weights = paths[name]
+        s = make_solver(tecno, n, prec, rec, mlp, weights)
+        s.set_state(spun)
+        if args.warmup:
+            s.run_steps(args.warmup, final_time=1.0, t_stamps=1)
+        res[name] = time_steps(s, n, args.steps)
+        if rec == "sp_weno_dl" and args.only in ("", "matrix"):
+            finals[name] = s.get_state()
+        s.close()
+        print(f"# {name} from the spun-up state (t={t_spun:.4f}): {res[name]}", file=sys.stderr,
+              flush=True)
+    if args.only and args.only != "matrix":
         return
     h = res["headline"]
     b = res["fp64_spweno"]
@@ -559,6 +626,25 @@ def main():
     bytes_per_launch = n * n * 8 * NVAR * esize / 3.0
     achieved = bytes_per_launch / (h["stage_ms"] * 1e-3) / 1e9
     traffic = load_traffic("headline")
+    valu = load_profile("valu", "headline")
+    mfma = load_profile("mfma", "headline")
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "kernel": "stage_kernel<Euler2Mixed<DSP,BF16>>", "stage_ms": h["stage_ms"],
+            "bytes_per_launch_alg": bytes_per_launch}
+    if valu:
+        # wave64 VALU instructions per stage launch from the committed counter
+        # profile of the same state, divided by this run's measured stage time:
+        # the issue roof is 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+        rate = valu["valu_insts_per_dispatch"] / (h["stage_ms"] * 1e-3)
+        roof["valu"] = {"achieved": rate, "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
+                        "frac": rate / VALU_PEAK, "insts_per_launch": valu["valu_insts_per_dispatch"],
+                        "source": "profiles/valu_headline.json"}
+        if rate / VALU_PEAK > roof["frac"]:
+            roof["bound"] = "valu"
+    if mfma:
+        roof["mfma"] = {k: mfma[k] for k in mfma if k != "command"}
     out = {
         "metric": METRIC,
         "value": h["cell_updates_per_s"],
@@ -579,30 +665,49 @@ def main():
                    "precision": "fp64 state, cell records, entropy-conservative flux and "
                                 "eigen-projection; fp32 jump-form reconstruction and dissipation; "
                                 "DSP-WENO network as bf16 MFMA 32x32x16 with fp32 accumulation",
-                   "timed_from_time": t_spun},
+                   "spinup_steps": SPINUP_STEPS, "timed_from_time": t_spun,
+                   "sign_fallback": 1},
         "fp64_spweno_cell_updates_per_s": b["cell_updates_per_s"],
         "speedup_vs_fp64_spweno": h["cell_updates_per_s"] / b["cell_updates_per_s"],
         "fp64_dsp_cell_updates_per_s": res["fp64_dsp"]["cell_updates_per_s"],
         "speedup_vs_fp64_dsp": h["cell_updates_per_s"] / res["fp64_dsp"]["cell_updates_per_s"],
         "sign_fallbacks_per_step": h["sign_fallbacks"] / max(args.steps, 1),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "stage_kernel<Euler2Mixed<DSP,BF16>>",
-                     "stage_ms": h["stage_ms"], "bytes_per_launch_alg": bytes_per_launch},
+        "roofline": roof,
     }
-    valu = load_profile("valu", "headline")
-    if valu:
-        # instructions per stage dispatch from the committed counter profile,
-        # divided by this run's measured stage time
-        rate = valu["valu_insts_per_dispatch"] / (h["stage_ms"] * 1e-3)
-        out["valu_roofline"] = {"achieved": rate, "peak": VALU_PEAK,
-                                "unit": "wave64 VALU instructions/s", "frac": rate / VALU_PEAK,
-                                "insts_per_launch": valu["valu_insts_per_dispatch"]}
-    out["l1_gate"] = l1_gate(tecno)
-    print(f"# equal-error gate: {out['l1_gate']}", file=sys.stderr, flush=True)
+    # error of the headline path against the FP64 DSP-WENO path (the same
+    # method in the reference's arithmetic) on the KH state after the same
+    # warmup + timed steps from the same state
+    out["deviation_vs_fp64_dsp"] = state_deviation(finals["headline"], finals["fp64_dsp"],
+                                                   args.warmup + args.steps)
+    # every network variant against the FP64 path with the same weights, from
+    # the same state after the same number of steps
+    matrix = {}
+    for name in ("headline", "mixed_fp16_mfma", "mixed_fp32_simt", "mixed_bf16_mfma_xavier",
+                 "fp64_dsp", "fp64_dsp_xavier"):
+        ref = "fp64_dsp_xavier" if name.endswith("xavier") else "fp64_dsp"
+        row = {"network": {tecno.MLP_BF16: "bf16 MFMA", tecno.MLP_FP16: "fp16 MFMA",
+                           tecno.MLP_FP32: "fp32 SIMT", tecno.MLP_FP64: "fp64 SIMT"}[paths[name][2]],
+               "weights": paths[name][3], "stage_ms": res[name]["stage_ms"],
+               "cell_updates_per_s": res[name]["cell_updates_per_s"],
+               "sign_fallbacks_per_step": res[name]["sign_fallbacks"] / max(args.steps, 1)}
+        if name != ref:
+            row["deviation_vs_fp64"] = state_deviation(finals[name], finals[ref],
+                                                       args.warmup + args.steps)
+        matrix[name] = row
+    out["network_matrix"] = matrix
+    if args.only == "matrix":
+        print(json.dumps(out["network_matrix"]), flush=True)
+        return
+    gate = l1_gate(tecno)
+    out["l1_gate"] = gate
+    # the metric's condition: L1 within 2% of the FP64 SP-WENO path on every
+    # mesh of the vortex and the same convergence order (to 0.05)
+    out["equal_l1_vs_fp64_spweno"] = gate["pass_vs_fp64_spweno"]
+    out["equal_l1_vs_fp64_dsp"] = gate["pass_vs_fp64_dsp"]
+    print(f"# equal-error gate: {gate}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline_all_cores"] = cpu_baseline_all_cores()
     print(json.dumps(out), flush=True)
 
 

@@ -37,7 +37,9 @@ extern "C" int tecno_read_param(const char* path, int level, tecno_cfg* cfg, int
       std::memcpy(net_file, p.net_file.c_str(), p.net_file.size() + 1);
     }
     *cfg = c;
-    if (n_levels) *n_levels = (int)p.Nx.size();
+    // mesh levels FV::run runs: all with OOC_study yes, else the first
+    // (parameter.cc:636-639); `level` may still address any listed mesh
+    if (n_levels) *n_levels = p.mesh_levels;
     if (final_time) *final_time = p.final_time;
     if (time_stamps) *time_stamps = p.t_stamps;
     if (max_iter) *max_iter = p.max_iter;

@@ -246,7 +246,8 @@ int main(int argc, char** argv) {
                 << "\nmodel " << p.base.model << "\nrecon " << p.base.recon << " "
                 << p.base.spweno_corr << " " << p.base.eno_level << "\ntime_scheme "
                 << p.base.time_scheme << "\nreconstruction "
-                << p.reconstruction << "\nlevels " << p.Nx.size() << "\nfinal_time "
+                << p.reconstruction << "\nlevels " << p.mesh_levels << "\nlisted " << p.Nx.size()
+                << "\nfinal_time "
                 << p.final_time << "\nt_stamps " << p.t_stamps << "\nooc " << p.ooc
                 << "\npdim " << p.pdim << "\ncfl " << p.base.cfl << "\ngamma " << p.base.gamma
                 << "\nmu " << p.base.mu_model << " " << p.base.mu_ref << "\nbc";

@@ -140,7 +140,9 @@ void tecno_cfg_default(tecno_cfg* cfg);
  * files): parse a param.in file into the tecno_cfg of mesh level `level`
  * (dims from problem_dim, at least 2 for systems; n, lo, hi of that level;
  * precision FP64 and the fix flags 0 - set them afterwards).  Optional
- * outputs (NULL to skip): the number of mesh levels, final_time,
+ * outputs (NULL to skip): the number of mesh levels FV::run solves (every
+ * listed mesh with OOC_study yes, else 1: parameter.cc:636-639; `level` may
+ * address any listed mesh), final_time,
  * time_stamps, max_iter, and net_file copied into a buffer of net_len
  * bytes.  A malformed file or an unknown name returns TECNO_EINVAL with the
  * reader's message. */

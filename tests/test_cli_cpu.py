@@ -212,6 +212,25 @@ def test_library_param_reader_errors(tmp_path):
     assert b"null argument" in tecno.lib().tecno_last_error()
 
 
+def test_mesh_levels_without_ooc(tmp_path):
+    """several meshes listed with OOC_study no: FV::run solves only the first
+    (parameter.cc:636-639), so the reader reports one level, while every
+    listed mesh can still be read"""
+    import tecno
+    src = open(os.path.join(CASES, "vortex_spweno_ooc.param.in")).read()
+    assert "OOC_study   yes" in src
+    p = tmp_path / "two_meshes.param.in"
+    p.write_text(src.replace("OOC_study   yes", "OOC_study   no"))
+    cfg, info = tecno.read_param(p)
+    assert info["levels"] == 1
+    cfg1, _ = tecno.read_param(p, level=1)
+    assert (cfg.n[0], cfg1.n[0]) == (32, 64)
+    out, _ = dry(str(p))
+    assert int(out["levels"][0]) == 1 and int(out["listed"][0]) == 2
+    _, info_ooc = tecno.read_param(os.path.join(CASES, "vortex_spweno_ooc.param.in"))
+    assert info_ooc["levels"] == 2
+
+
 REF_EXAMPLES = sorted(glob.glob("/root/reference/*/examples/**/param.in", recursive=True))
 
 

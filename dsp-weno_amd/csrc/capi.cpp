@@ -13,6 +13,7 @@
 
 #include "../../include/tecno.h"
 #include "fv_launch.hpp"
+#include "diag.hpp"
 #include "ensemble.hpp"
 #include "mlp.hpp"
 
@@ -242,6 +243,15 @@ static int upload_net(tecno_ctx* c) {
 }
 
 static bool fp64(const tecno_ctx* c) { return c->cfg.precision == TECNO_FP64; }
+
+// cells the reference stores per held cell: the param.in cell counts of the
+// collapsed axes (a "2D" run holds Nx x Ny x Nz_cell = 2 cells in the
+// reference), so sums over held cells times this are the reference's sums
+static double collapsed_cells(const tecno_ctx* c) {
+  double l = 1.0;
+  for (int d = c->dims; d < 3; ++d) l *= c->cfg.n[d];
+  return l;
+}
 
 static hipError_t launch_stage(tecno_ctx* c, const void* U, const void* U0, void* Uo,
                                const StageCoef& k) {
@@ -761,7 +771,7 @@ int tecno_audit_state(tecno_ctx* c, tecno_audit* out) {
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int b = 0; b < nblk; ++b)
     for (int v = 0; v < 9; ++v) acc[v] = (v == 7) ? std::max(acc[v], part[b * 9 + v]) : acc[v] + part[b * 9 + v];
-  const double dV = c->vol;
+  const double dV = c->vol * collapsed_cells(c);
   out->total_entropy = acc[0] * dV;
   out->kinetic_energy = 0.5 * acc[1] * dV;
   out->mass = acc[2] * dV;
@@ -794,8 +804,162 @@ int tecno_l1_error(tecno_ctx* c, const double* exact, int64_t n, double* l1) {
   for (int v = 0; v < c->nprim; ++v) {
     double s = 0;
     for (int b = 0; b < nblk; ++b) s += hp[b * 5 + v];
-    l1[v] = s * c->vol;
+    l1[v] = s * c->vol * collapsed_cells(c);
   }
+  return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ globals
+static DiagGrid diag_grid(const tecno_ctx* c) {
+  DiagGrid g{};
+  g.dims = c->dims;
+  for (int d = 0; d < 3; ++d) {
+    g.n[d] = c->n[d];
+    g.h[d] = c->h[d];
+  }
+  for (int i = 0; i < 6; ++i) g.bc[i] = c->cfg.bc[i];
+  g.N = c->N;
+  g.fix_edges = c->cfg.fix_edges;
+  return g;
+}
+
+static int64_t diag_nodes_held(const tecno_ctx* c) {
+  return (int64_t)(c->n[0] + 1) * (c->n[1] + 1) * (c->dims == 3 ? c->n[2] + 1 : 3);
+}
+
+// the 2D node plane K (0..Nz of the reference's storage) is held as layer
+// 0 (K = 0), 2 (K = Nz) or 1 (in between)
+static int plane_layer(int K, int Nz) { return K == 0 ? 0 : (K == Nz ? 2 : 1); }
+
+// nodal diagnostics into the scratch buffer: omega (3 x held nodes) when
+// with_omega, then the per-block partial sums; *part_out points at them
+static int run_nodes(tecno_ctx* c, bool with_omega, double** omega_out, double** part_out,
+                     int* nblk) {
+  const int64_t NN = diag_nodes_held(c);
+  const int64_t nb = (NN + 63) / 64;
+  const size_t need = (size_t)(with_omega ? 3 * NN : 0) + (size_t)nb * 4 + 64;
+  if (int rc = ensure_scratch(c, need * sizeof(double))) return rc;
+  double* om = with_omega ? c->scratch : nullptr;
+  double* part = c->scratch + (with_omega ? 3 * NN : 0);
+  const bool ns = c->cfg.model == TECNO_MODEL_NS_ES || c->cfg.model == TECNO_MODEL_NS_KEP;
+  HIPCHK(diag_nodes(c->stream, diag_grid(c), (const double*)c->U[0], c->cfg.gamma, c->P64,
+                    ns && c->dims == 3, om, part, nblk));
+  *omega_out = om;
+  *part_out = part;
+  return 0;
+}
+
+extern "C" {
+
+int tecno_compute_globals(tecno_ctx* c, tecno_globals* out) {
+  if (!c || !out) return fail(TECNO_EINVAL, "null argument");
+  if (!c->state_set) return fail(TECNO_ESTATE, "no state");
+  if (int rc = need_net(c)) return rc;
+  tecno_audit a;
+  if (int rc = tecno_audit_state(c, &a)) return rc;
+  std::memset(out, 0, sizeof(*out));
+  Ctl h{};
+  HIPCHK(hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  out->time = h.time;
+  out->total_entropy = a.total_entropy;
+  if (c->cfg.system == TECNO_SYS_SCALAR) return 0;
+  out->kinetic_energy = a.kinetic_energy;
+  const double layers = collapsed_cells(c);
+  // KE_diss_pr: the inviscid residual of the state (the first stage's
+  // residual, fv.cc:756-764) from one stage launch that writes the residual
+  // itself: u = 0*u + 0*u0 - (dt/vol)*(-1)*res with dt = vol, exactly res.
+  // The control block (dt, counters) is restored afterwards.
+  {
+    Ctl m = h;
+    m.dt = c->vol;
+    m.active = 1;
+    HIPCHK(hipMemcpyAsync(c->ctl, &m, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+    const StageCoef k{0.0, 0.0, -1.0, 0.0, 0, 0};
+    hipError_t e;
+    if (fp64(c)) {
+      Phys<double> P = c->P64;
+      P.model = TECNO_MODEL_EULER;
+      e = Launch<double>::stage(c->L, P, c->U[0], c->U[0], c->U[1], k);
+    } else {
+      Phys<float> P = c->P32;
+      P.model = TECNO_MODEL_EULER;
+      e = Launch<float>::stage(c->L, P, c->U[0], c->U[0], c->U[1], k);
+    }
+    HIPCHK(e);
+    HIPCHK(hipMemcpyAsync(c->ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+  }
+  const int nbr = 1024;
+  if (int rc = ensure_scratch(c, (size_t)nbr * sizeof(double))) return rc;
+  HIPCHK(diag_ke_diss_pr(c->stream, diag_grid(c), (const double*)c->U[0], (const double*)c->U[1],
+                         c->scratch, nbr));
+  std::vector<double> pr(nbr);
+  HIPCHK(hipMemcpyAsync(pr.data(), c->scratch, nbr * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double kdp = 0.0;
+  for (double v : pr) kdp += v;
+  out->ke_diss_pr = kdp * layers;
+  // nodal sums: enstrophy (compute_global.cc:32-75) and KE_diss_stress
+  double *om, *part;
+  int nblk = 0;
+  if (int rc = run_nodes(c, false, &om, &part, &nblk)) return rc;
+  std::vector<double> hp((size_t)nblk * 4);
+  HIPCHK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double S[4] = {0, 0, 0, 0};
+  for (int b = 0; b < nblk; ++b)
+    for (int q = 0; q < 4; ++q) S[q] += hp[(size_t)b * 4 + q];
+  const tecno_cfg& g = c->cfg;
+  // nodal volume: (extent / (cells + 1)) per axis, the reference's cell
+  // counts (compute_global.cc:70-72)
+  double nvol = 1.0;
+  for (int d = 0; d < 3; ++d) nvol *= (g.hi[d] - g.lo[d]) / (g.n[d] + 1);
+  double ens = S[0];
+  if (c->dims == 2) {
+    // every z node plane of the reference's storage, with its weight
+    const int Nz = g.n[2];
+    ens = 0.0;
+    for (int K = 0; K <= Nz; ++K) {
+      double w = 1.0;
+      if (K == 0 && g.bc[4] == TECNO_BC_PERIODIC) w /= 2.0;
+      if (K == Nz && g.bc[5] == TECNO_BC_PERIODIC) w /= 2.0;
+      ens += w * S[plane_layer(K, Nz)];
+    }
+  }
+  out->enstrophy = 0.5 * ens * nvol;
+  out->ke_diss_stress = S[3];
+  return 0;
+}
+
+int tecno_vorticity(tecno_ctx* c, double* omega, int64_t n_nodes) {
+  if (!c || !omega) return fail(TECNO_EINVAL, "null argument");
+  if (c->cfg.system == TECNO_SYS_SCALAR) return fail(TECNO_EINVAL, "vorticity: systems solver only");
+  if (!c->state_set) return fail(TECNO_ESTATE, "no state");
+  const tecno_cfg& g = c->cfg;
+  const int nx = c->n[0] + 1, ny = c->n[1] + 1;
+  const int nzr = (c->dims == 3 ? c->n[2] : g.n[2]) + 1;
+  const int64_t NNr = (int64_t)nx * ny * nzr;
+  if (n_nodes != NNr) return fail(TECNO_EINVAL, "node count mismatch");
+  double *om, *part;
+  int nblk = 0;
+  if (int rc = run_nodes(c, true, &om, &part, &nblk)) return rc;
+  const int64_t NN = diag_nodes_held(c);
+  std::vector<double> h((size_t)3 * NN);
+  HIPCHK(hipMemcpyAsync(h.data(), om, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->dims == 3) {
+    std::memcpy(omega, h.data(), h.size() * sizeof(double));
+    return 0;
+  }
+  const int64_t plane = (int64_t)nx * ny;
+  for (int d = 0; d < 3; ++d)
+    for (int K = 0; K < nzr; ++K)
+      std::memcpy(omega + d * NNr + K * plane, h.data() + d * NN + plane_layer(K, nzr - 1) * plane,
+                  plane * sizeof(double));
   return 0;
 }
 

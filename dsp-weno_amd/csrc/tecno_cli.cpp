@@ -7,10 +7,16 @@
 //   sol####.vtk     the solution at t = 0 and at every output stamp
 //                   (output.cc:10-90 layout; systems: density, velocity,
 //                   pressure, physical_entropy; scalar: solution)
-//   1Dsol####.dat   scalar runs: x and u along the first row
-//   globals.dat     per step: time, total entropy, its relative change,
-//                   kinetic energy (systems) — compute_global.cc:10-89; the
-//                   dissipation-rate and enstrophy columns are written as 0
+//   omg####.vtk     systems: the nodal vorticity with every sol file
+//                   (output.cc:131-160, find_vort.cc:9-29)
+//   1Dsol####.dat   along the first row: x and u (scalar, enflo_scalar
+//                   output.cc:64-66), x, rho, u, p (systems, output.cc:164-169)
+//   globals.dat     at the start of every step, on every mesh level: time,
+//                   total entropy, its relative change and, for systems,
+//                   kinetic energy, KE_diss_pr, KE_diss_stress, enstrophy
+//                   (compute_global.cc:10-89, evaluated as the reference does
+//                   at the first stage; the stream's precision as the
+//                   reference sets it)
 //   l1_err.dat, l2_err.dat, linf_err.dat   OOC tables (compute_ooc.cc)
 // A 2D or 1D problem is solved on its active axes; files repeat the
 // collapsed layers the reference stores (Nz = 2 for "2D"), and the error
@@ -108,15 +114,67 @@ void write_vtk(const Run& r, const std::string& dir, int counter, double time) {
   });
 }
 
-void write_1d(const Run& r, const std::string& dir, int counter) {
-  std::ofstream f(dir + "/1Dsol" + counter_name(counter) + ".dat");
-  for (int i = 0; i < r.Nx; ++i)
-    f << (r.p->xmin + r.dx / 2) + r.dx * i << "\t" << r.prim[(size_t)i] << "\t" << "\n";
-}
-
 void fetch(Run& r) {
   check(tecno_get_state(r.c, r.prim.data(), tecno_num_cells(r.c)), "tecno_get_state");
 }
+
+void write_1d(const Run& r, const std::string& dir, int counter) {
+  std::ofstream f(dir + "/1Dsol" + counter_name(counter) + ".dat");
+  const int64_t N = (int64_t)r.ax * r.ay * r.az;
+  for (int i = 0; i < r.Nx; ++i) {
+    f << (r.p->xmin + r.dx / 2) + r.dx * i << "\t";
+    if (r.p->scalar)
+      f << r.prim[(size_t)i] << "\t";
+    else
+      f << r.prim[(size_t)i] << "\t" << r.prim[(size_t)N + i] << "\t"
+        << r.prim[(size_t)(r.nprim - 1) * N + i];
+    f << "\n";
+  }
+}
+
+// output.cc:131-160: the nodal vorticity, (k, j, i) order
+void write_omg(const Run& r, const std::string& dir, int counter, double time) {
+  const Param& p = *r.p;
+  const int64_t nn = (int64_t)(r.Nx + 1) * (r.Ny + 1) * (r.Nz + 1);
+  std::vector<double> w((size_t)3 * nn);
+  check(tecno_vorticity(r.c, w.data(), nn), "tecno_vorticity");
+  std::ofstream vtk(dir + "/omg" + counter_name(counter) + ".vtk");
+  vtk << "# vtk DataFile Version 3.0\nenflo\nASCII\nDATASET STRUCTURED_POINTS\n";
+  vtk << "FIELD FieldData 1\nTIME 1 1 double\n" << time << "\n";
+  vtk << "DIMENSIONS " << (r.Nx + 1) << " " << (r.Ny + 1) << " " << (r.Nz + 1) << "\n";
+  vtk << "SPACING " << r.dx << " " << r.dy << " " << r.dz << " \n";
+  vtk << "ORIGIN " << p.xmin << " " << p.ymin << " " << p.zmin << " \n";
+  vtk << "POINT_DATA  " << nn << "\nVECTORS vorticity float\n";
+  for (int64_t q = 0; q < nn; ++q) vtk << w[q] << " " << w[nn + q] << " " << w[2 * nn + q] << "\n";
+}
+
+// one solution output (FV::output, output.cc:10-172)
+void write_outputs(Run& r, const std::string& dir, int counter, double time) {
+  fetch(r);
+  write_vtk(r, dir, counter, time);
+  if (!r.p->scalar) write_omg(r, dir, counter, time);
+  write_1d(r, dir, counter);
+}
+
+// one globals.dat line (compute_global.cc:80-88; enflo_scalar
+// compute_global.cc:34-37).  The reference writes the time at the stream's
+// current precision and then sets 10 digits, which the stream keeps: the
+// first line's time has 6 significant digits, every later value 10.
+void write_globals(std::ofstream& glob, const Param& p, tecno_ctx* c, double& ent0) {
+  tecno_globals g;
+  check(tecno_compute_globals(c, &g), "globals");
+  if (g.time == 0) {
+    ent0 = g.total_entropy;
+    if (ent0 == 0) ent0 = 1.0;  // compute_global.cc:63-67
+  }
+  glob << g.time << " " << std::setprecision(10) << g.total_entropy << " "
+       << (g.total_entropy - ent0) / (p.scalar ? ent0 : std::fabs(ent0)) << " ";
+  if (!p.scalar)
+    glob << g.kinetic_energy << " " << g.ke_diss_pr << " " << g.ke_diss_stress << " "
+         << g.enstrophy << " ";
+  glob << std::endl;
+}
+
 
 struct Err {
   double l1[5] = {0, 0, 0, 0, 0}, l2[5] = {0, 0, 0, 0, 0}, linf[5] = {0, 0, 0, 0, 0};
@@ -271,6 +329,7 @@ int main(int argc, char** argv) {
     std::filesystem::remove_all(dir);
     std::filesystem::create_directories(dir);
     std::ofstream glob(dir + "/globals.dat");
+    double ent0 = 1.0;  // the reference's total_ent0, set at time 0 of each level
     std::vector<Err> errs;
     for (int rl = 0; rl < p.mesh_levels; ++rl) {
       Run r;
@@ -330,41 +389,24 @@ int main(int argc, char** argv) {
       const auto t0 = std::chrono::steady_clock::now();
       int iters = 0;
       double time = 0.0;
-      if (p.ooc) {
-        check(tecno_advance(r.c, p.final_time, p.t_stamps, p.max_iter, &iters, &time), "advance");
-      } else {
-        // one step at a time: globals every step, solution at every stamp
-        int counter = 0;
-        fetch(r);
-        write_vtk(r, dir, counter, 0.0);
-        if (p.scalar && r.dims == 1) write_1d(r, dir, counter);
-        ++counter;
-        const double t_int = p.final_time / p.t_stamps;
-        double t_save = t_int;
-        double ent0 = 0.0;
-        bool first = true;
-        while (time < p.final_time && iters < p.max_iter) {
-          tecno_audit a;
-          check(tecno_audit_state(r.c, &a), "audit");
-          const double ent = (p.scalar ? a.kinetic_energy : a.total_entropy) * r.layers;
-          if (first) {
-            ent0 = ent;
-            if (ent0 == 0.0) ent0 = 1.0;  // compute_global.cc:63-67
-            first = false;
-          }
-          glob << time << " " << std::setprecision(10) << ent << " "
-               << (ent - ent0) / (p.scalar ? ent0 : std::fabs(ent0)) << " ";
-          if (!p.scalar) glob << a.kinetic_energy * r.layers << " " << 0.0 << " " << 0.0 << " " << 0.0 << " ";
-          glob << std::endl << std::setprecision(6);
-          check(tecno_run_steps(r.c, 1, p.final_time, p.t_stamps), "step");
-          check(tecno_get_time(r.c, &time, &iters), "time");
-          if (time == t_save) {
-            fetch(r);
-            write_vtk(r, dir, counter, time);
-            if (p.scalar && r.dims == 1) write_1d(r, dir, counter);
-            ++counter;
-            t_save += t_int;
-          }
+      // FV::run (fv.cc:1231-1324): the solution files at t = 0 and at every
+      // stamp unless OOC_study is on (print_soln, fv.h:50-63); globals on
+      // every mesh level, at the start of every step
+      // (systems); the scalar solver writes them at t = 0 and after every
+      // step, the final state included (enflo_scalar fv.cc:522-607)
+      int counter = 0;
+      if (!p.ooc) write_outputs(r, dir, counter++, 0.0);
+      if (p.scalar) write_globals(glob, p, r.c, ent0);
+      const double t_int = p.final_time / p.t_stamps;
+      double t_save = t_int;
+      while (time < p.final_time && iters < p.max_iter) {
+        if (!p.scalar) write_globals(glob, p, r.c, ent0);
+        check(tecno_run_steps(r.c, 1, p.final_time, p.t_stamps), "step");
+        check(tecno_get_time(r.c, &time, &iters), "time");
+        if (p.scalar) write_globals(glob, p, r.c, ent0);
+        if (time == t_save) {
+          if (!p.ooc) write_outputs(r, dir, counter++, time);
+          t_save += t_int;
         }
       }
       const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -123,6 +123,12 @@ def make_cfg(*, system="euler", flux="kepes_tecno4_roe", recon="sp_weno", n=(32,
     return c
 
 
+class Globals(C.Structure):
+    _fields_ = [("time", C.c_double), ("total_entropy", C.c_double),
+                ("kinetic_energy", C.c_double), ("ke_diss_pr", C.c_double),
+                ("ke_diss_stress", C.c_double), ("enstrophy", C.c_double)]
+
+
 class EnsGroup(C.Structure):
     _fields_ = [("precision", C.c_int), ("dims", C.c_int), ("n_members", C.c_int),
                 ("n_tiles", C.c_int), ("grid", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int),
@@ -174,6 +180,8 @@ def lib():
         L.tecno_synchronize.argtypes = [P]
         L.tecno_audit_state.argtypes = [P, C.POINTER(Audit)]
         L.tecno_l1_error.argtypes = [P, dp, I64, dp]
+        L.tecno_compute_globals.argtypes = [P, C.POINTER(Globals)]
+        L.tecno_vorticity.argtypes = [P, dp, I64]
         L.tecno_reconstruct_batch.argtypes = [P, dp, I64, dp, dp]
         L.tecno_flux_batch.argtypes = [P, dp, dp, I64, I32, dp]
         L.tecno_mlp_forward_batch.argtypes = [P, dp, I64, dp, dp]
@@ -305,6 +313,22 @@ class Solver:
         a = Audit()
         _check(lib().tecno_audit_state(self.h, C.byref(a)))
         return a
+
+    def globals(self):
+        """FV::compute_global of the current state (tecno_compute_globals)"""
+        g = Globals()
+        _check(lib().tecno_compute_globals(self.h, C.byref(g)))
+        return g
+
+    def vorticity(self):
+        """nodal vorticity (3, Nz+1, Ny+1, Nx+1) over the reference's nodes
+        (a 2D grid: Nz = n[2] of its configuration)"""
+        c = self.cfg
+        nx, ny = c.n[0] + 1, (c.n[1] + 1) if c.dims >= 2 else 2
+        nz = c.n[2] + 1
+        out = np.zeros(3 * nx * ny * nz)
+        _check(lib().tecno_vorticity(self.h, _dp(out), out.size // 3))
+        return out.reshape(3, nz, ny, nx)
 
     def l1_error(self, exact):
         exact = np.ascontiguousarray(exact, dtype=np.float64).reshape(-1)

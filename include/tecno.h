@@ -219,9 +219,48 @@ typedef struct tecno_audit {
 int tecno_audit_state(tecno_ctx* ctx, tecno_audit* out);
 
 /* L1 error of the primitive state against exact values (compute_error,
- * enflo/src/compute_errors.cc:9-34): out[v] = sum |exact-prim| * dV. */
+ * enflo/src/compute_errors.cc:9-34): out[v] = sum |exact-prim| * dV.
+ * Sums here and in tecno_audit_state are the reference's sums: a cell of
+ * a collapsed axis (a 2D grid's z, a 1D grid's y and z) stands for the
+ * n[d] cells the reference stores there, so dV uses that axis's extent. */
 int tecno_l1_error(tecno_ctx* ctx, const double* exact_prim, int64_t n_cells,
                    double* l1_out);
+
+/* FV::compute_global (enflo/src/compute_global.cc:10-89;
+ * enflo_scalar/src_scalar/compute_global.cc:10-38) for the current state,
+ * the values of one globals.dat line (the reference evaluates them at the
+ * first stage of each step, i.e. for the state the step starts from):
+ *   total_entropy  sum -rho s/(gamma-1) dV (scalar: sum u^2/2 dV)
+ *   kinetic_energy 0.5 sum rho|u|^2 dV
+ *   ke_diss_pr     fv.cc:756-764: -sum over cells of (|u|^2/2 R_mass +
+ *                  u . R_momentum), R the inviscid residual of the state
+ *   ke_diss_stress fv.cc:792-1156: Navier-Stokes only (0 otherwise), minus
+ *                  the node sum of (grad u . viscous momentum flux) h w, edge
+ *                  nodes of two periodic planes twice and the last two
+ *                  corners' derivatives and weights taken at node
+ *                  (i, Nz, Nz), as the reference's loops do (fix_edges:
+ *                  each node once, at itself)
+ *   enstrophy      compute_global.cc:32-75: 0.5 * nodal volume * node sum of
+ *                  (8-cell mean density) |vorticity|^2 w
+ * Systems solver: one extra stage launch (the residual) and one node
+ * pass. */
+typedef struct tecno_globals {
+  double time;
+  double total_entropy;
+  double kinetic_energy;
+  double ke_diss_pr;
+  double ke_diss_stress;
+  double enstrophy;
+} tecno_globals;
+int tecno_compute_globals(tecno_ctx* ctx, tecno_globals* out);
+
+/* FV::find_vorticity (enflo/src/find_vort.cc:9-29): the vorticity at the
+ * reference's (Nx+1)(Ny+1)(Nz+1) nodes (a 2D grid: Nz = n[2] as given, the
+ * node planes of its z-uniform storage), omega[c*n_nodes + (k*(Ny+1) + j)*
+ * (Nx+1) + i] for components c = x, y, z.  Ghost cells as the reference
+ * fills them (apply_bc, fv.cc:121-361: edge and corner ghosts periodic).
+ * Systems solver only. */
+int tecno_vorticity(tecno_ctx* ctx, double* omega, int64_t n_nodes);
 
 /* ---- batched seams (test / kernel-level entry points) ------------------ */
 /* FV::reconstruct on n stencils of 2*ng values (ng = 2, or k for eno k):

@@ -19,6 +19,18 @@
 //          states (systems only; via the entropy_var array).
 //   time  <param.in> <nsteps>   runs nsteps steps of the time loop and prints
 //          the wall time of the steps alone (initialisation excluded).
+//   globals <param.in> <nsteps> <out.bin>   FV::run's time loop with its
+//          diagnostics: find_vorticity before the loop and after every step,
+//          compute_global at the first stage of every step (fv.cc:1244-1306).
+//          header int32[4] = {nx, ny, nz, nsteps}; per step {time, total_ent,
+//          KE, KE_diss_pr, KE_diss_stress, Enstrophy} (scalar: {time,
+//          total_ent}); systems then the nodal vorticity before the first and
+//          after the last step, 3 x (nx+1)(ny+1)(nz+1) each, x fastest.
+//          KE_diss_stress is set to 0 before the loop: the Euler model never
+//          assigns it (fv.cc:591-601) and prints whatever the member holds.
+//   full  <param.in>   FV::run itself (the reference's own driver, main.cc:23-24)
+//          with the same KE_diss_stress initialisation; writes the
+//          reference's output files under the param.in's output_path.
 
 #include <cmath>
 #include <cstdio>
@@ -159,6 +171,112 @@ static int mode_run(char* param, int nsteps, const char* out) {
   hdr[4] = done;
   fwrite(hdr, sizeof hdr, 1, f);
   fclose(f);
+  return 0;
+}
+
+#ifndef HARNESS_SCALAR
+static void dump_vorticity(FV& fv, FILE* f) {
+  const int nx = fv.Nx_cell + 1, ny = fv.Ny_cell + 1, nz = fv.Nz_cell + 1;
+  std::vector<double> buf((size_t)3 * nx * ny * nz);
+  const size_t NN = (size_t)nx * ny * nz;
+  size_t q = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i, ++q) {
+        const Vector& w = fv.vorticity[i][j][k];
+        buf[q] = w.x;
+        buf[NN + q] = w.y;
+        buf[2 * NN + q] = w.z;
+      }
+  write_bin(f, buf.data(), buf.size());
+}
+#endif
+
+static int mode_globals(char* param, int nsteps, const char* out) {
+  FV fv(param);
+  setup(fv);
+#ifndef HARNESS_SCALAR
+  fv.prim_to_con();
+  fv.prim_to_ent();
+  fv.find_vorticity();
+  fv.KE_diss_stress = 0.0;
+#endif
+  fv.time = 0.0;
+  fv.iter = 0;
+  std::vector<double> rows;
+#ifndef HARNESS_SCALAR
+  std::vector<double> vort0;
+  {
+    char* mem = nullptr;
+    size_t len = 0;
+    FILE* m = open_memstream(&mem, &len);
+    dump_vorticity(fv, m);
+    fclose(m);
+    vort0.assign((double*)mem, (double*)(mem + len));
+    free(mem);
+  }
+#endif
+  double t_int = fv.param.final_time / fv.param.t_stamps;
+  double t_save = t_int;
+  int done = 0;
+  while (done < nsteps && fv.time < fv.param.final_time && fv.iter < fv.param.max_iter) {
+#ifdef HARNESS_SCALAR
+    fv.solution_old = fv.solution;
+#else
+    fv.conserved_old = fv.conserved;
+#endif
+    fv.compute_dt();
+    if (fv.time + fv.dt > t_save) fv.dt = t_save - fv.time;
+    clear_residual2(fv);
+    for (int rk = 0; rk < fv.param.n_rks; ++rk) {
+#ifdef HARNESS_SCALAR
+      fv.compute_residual(rk);
+      if (rk == 0) {
+        fv.compute_global();
+        rows.push_back(fv.time);
+        rows.push_back(fv.total_ent);
+      }
+      fv.update_solution(rk);
+      fv.apply_bc();
+#else
+      fv.prim_to_ent();
+      fv.compute_residual(rk);
+      if (rk == 0) {
+        fv.compute_global();
+        const double r[6] = {fv.time, fv.total_ent, fv.KE, fv.KE_diss_pr, fv.KE_diss_stress,
+                             fv.Enstrophy};
+        rows.insert(rows.end(), r, r + 6);
+      }
+      fv.update_solution(rk);
+      fv.con_to_prim();
+#endif
+    }
+    fv.time += fv.dt;
+    ++fv.iter;
+#ifndef HARNESS_SCALAR
+    fv.find_vorticity();
+#endif
+    if (fv.time == t_save) t_save += t_int;
+    ++done;
+  }
+  FILE* f = fopen(out, "wb");
+  int32_t hdr[4] = {fv.Nx_cell, fv.Ny_cell, fv.Nz_cell, done};
+  fwrite(hdr, sizeof hdr, 1, f);
+  write_bin(f, rows.data(), rows.size());
+#ifndef HARNESS_SCALAR
+  write_bin(f, vort0.data(), vort0.size());
+  dump_vorticity(fv, f);
+#endif
+  fclose(f);
+  return 0;
+}
+
+static int mode_full(char* param) {
+  FV fv(param);
+#ifndef HARNESS_SCALAR
+  fv.KE_diss_stress = 0.0;
+#endif
+  fv.run();
   return 0;
 }
 
@@ -310,6 +428,8 @@ int main(int argc, char** argv) {
   std::cout.setstate(std::ios_base::failbit); /* silence the solver's log */
   std::string m = argv[1];
   if (m == "time" && argc == 4) return mode_time(argv[2], atoi(argv[3]));
+  if (m == "globals" && argc == 5) return mode_globals(argv[2], atoi(argv[3]), argv[4]);
+  if (m == "full" && argc == 3) return mode_full(argv[2]);
   if (m == "run" && argc == 5) return mode_run(argv[2], atoi(argv[3]), argv[4]);
   if (m == "recon" && argc == 7)
     return mode_recon(argv[2], atoi(argv[3]), atoi(argv[4]), argv[5], argv[6]);

@@ -1581,8 +1581,14 @@ int tor_advance(tor_solver* s, double final_time, int t_stamps, int max_iter,
   return 0;
 }
 
+/* compute_global compute_global.cc:18-27,72-73 (scalar
+ * enflo_scalar/src_scalar/compute_global.cc:12-20): sums over the
+ * reference's cells; a collapsed axis (not swept outside faithful mode)
+ * holds one cell for the cfg->n[d] the reference stores */
 void tor_audit(const tor_solver* s, double out[8]) {
   double dV = s->g.h[0] * s->g.h[1] * s->g.h[2];
+  for (int d = 0; d < 3; ++d)
+    if (!s->active[d]) dV *= s->c.n[d];
   for (int v = 0; v < 8; ++v) out[v] = 0.0;
   double ent = 0.0, ke = 0.0, m = 0.0, mo[3] = {0, 0, 0}, e = 0.0, smax = 0.0;
   for (int i = 0; i < s->n[0]; ++i)

@@ -270,9 +270,9 @@ def test_linadv_sin4_l1_gpu():
     it, t = s.advance(final, stamps)
     assert it == 65 and t == 0.5
     x = -math.pi + (np.arange(400) + 0.5) * 2 * math.pi / 400
-    # the L1 of a collapsed run uses the param.in cell volume dx*dy*dz; the
-    # reference sums over its Ny*Nz = 4 duplicate cells, i.e. dx*dy*dz*4 = dx
-    l1 = s.l1_error(np.sin(x - t) ** 4)[0] * 4
+    # the L1 of a collapsed run counts the reference's Ny*Nz = 4 duplicate
+    # cells per held cell, as the reference's sum does (dx*dy*dz*4 = dx)
+    l1 = s.l1_error(np.sin(x - t) ** 4)[0]
     assert abs(l1 - 1.75268e-05) < 5e-11, l1
 
 

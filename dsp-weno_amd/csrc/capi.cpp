@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <random>
@@ -54,6 +55,7 @@ struct HostNet {
 
 struct tecno_ctx {
   tecno_cfg cfg;
+  int device = 0;  // the GPU current at tecno_create
   int dims = 2;
   int nvar = 4;    // conserved variables held
   int nprim = 4;   // primitive variables exchanged with the host
@@ -380,6 +382,10 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
     delete c;
     return fail(TECNO_EINVAL, "FP32 Euler path: kepes_tecno4_roe with sp_weno, sp_wenoc or "
                               "sp_weno_dl only (use FP64 for the other options)");
+  }
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return fail(TECNO_EHIP, "hipGetDevice failed");
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -754,6 +760,10 @@ int tecno_synchronize(tecno_ctx* c) {
   return 0;
 }
 
+}  // extern "C"
+static int finish_audit(tecno_ctx* c, const double acc[9], const Ctl& h, tecno_audit* out);
+extern "C" {
+
 int tecno_audit_state(tecno_ctx* c, tecno_audit* out) {
   if (!c || !out) return fail(TECNO_EINVAL, "null argument");
   if (!c->state_set) return fail(TECNO_ESTATE, "no state");
@@ -771,6 +781,90 @@ int tecno_audit_state(tecno_ctx* c, tecno_audit* out) {
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int b = 0; b < nblk; ++b)
     for (int v = 0; v < 9; ++v) acc[v] = (v == 7) ? std::max(acc[v], part[b * 9 + v]) : acc[v] + part[b * 9 + v];
+  return finish_audit(c, acc, h, out);
+}
+
+// ---- allocation-free audit
+// The partial sums of the audit's blocks and their reduced row live in a
+// file-scope device array (73.8 KB; every GPU the library runs on holds its
+// own copy until the process ends).  Host code takes the array's per-device
+// address from hipGetSymbolAddress and never uses its name.  Each call writes
+// every row it reads, so nothing has to start at zero; calls that overlap on
+// one GPU share the array and race.  The blocks stride over the cells, so
+// the array's size does not depend on the grid.
+}  // extern "C"
+
+namespace {
+constexpr int AUDIT_BLOCKS = 1024, AUDIT_ROW = 9, MAX_DEVICES = 64;
+}
+static __device__ __align__(256) double scratch[(AUDIT_BLOCKS + 1) * AUDIT_ROW];
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[MAX_DEVICES] = {};
+
+// the rows of the blocks reduced in block order, as tecno_audit_state sums
+// them on the host (max for the wave speed), into row AUDIT_BLOCKS
+__global__ void k_audit_rows(double* part, int nblk) {
+  const int v = threadIdx.x;
+  if (v >= AUDIT_ROW) return;
+  double acc = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    const double x = part[(int64_t)b * AUDIT_ROW + v];
+    acc = (v == 7) ? (acc < x ? x : acc) : acc + x;
+  }
+  part[(int64_t)nblk * AUDIT_ROW + v] = acc;
+}
+
+// this device's address of the array (first call per device, with the
+// device current)
+static int audit_scratch(int dev, double** out) {
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(TECNO_EINVAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(TECNO_EHIP, "hipGetSymbolAddress returned no address");
+    g_scratch_dev[dev] = (double*)p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+extern "C" {
+
+int tecno_audit_state_noalloc(tecno_ctx* c, tecno_audit* out) {
+  if (!c || !out) return fail(TECNO_EINVAL, "null argument");
+  if (!c->state_set) return fail(TECNO_ESTATE, "no state");
+  int cur = -1;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != c->device) HIPCHK(hipSetDevice(c->device));
+  double* part = nullptr;
+  int rc = audit_scratch(c->device, &part);
+  double acc[AUDIT_ROW];
+  Ctl h{};
+  if (rc == 0) {
+    hipError_t e = c->L.state_fp64
+                       ? StateOps<double>::audit(c->L, c->U[0], c->cfg.gamma, part, AUDIT_BLOCKS)
+                       : StateOps<float>::audit(c->L, c->U[0], c->cfg.gamma, part, AUDIT_BLOCKS);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_audit_rows, dim3(1), dim3(64), 0, c->stream, part, AUDIT_BLOCKS);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(acc, part + (size_t)AUDIT_BLOCKS * AUDIT_ROW, sizeof(acc),
+                         hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, c->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(TECNO_EHIP, std::string("audit: ") + hipGetErrorString(e));
+  }
+  if (cur != c->device) (void)hipSetDevice(cur);
+  if (rc) return rc;
+  return finish_audit(c, acc, h, out);
+}
+
+}  // extern "C"
+
+// the audit's sums (block rows reduced) to the reported quantities
+static int finish_audit(tecno_ctx* c, const double acc[9], const Ctl& h, tecno_audit* out) {
   const double dV = c->vol * collapsed_cells(c);
   out->total_entropy = acc[0] * dV;
   out->kinetic_energy = 0.5 * acc[1] * dV;
@@ -785,6 +879,8 @@ int tecno_audit_state(tecno_ctx* c, tecno_audit* out) {
   out->n_sign_fallback = (int64_t)h.nfallback;
   return 0;
 }
+
+extern "C" {
 
 int tecno_l1_error(tecno_ctx* c, const double* exact, int64_t n, double* l1) {
   if (!c || !exact || !l1) return fail(TECNO_EINVAL, "null argument");

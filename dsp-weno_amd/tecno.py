@@ -179,6 +179,7 @@ def lib():
         L.tecno_get_time.argtypes = [P, dp, C.POINTER(C.c_int)]
         L.tecno_synchronize.argtypes = [P]
         L.tecno_audit_state.argtypes = [P, C.POINTER(Audit)]
+        L.tecno_audit_state_noalloc.argtypes = [P, C.POINTER(Audit)]
         L.tecno_l1_error.argtypes = [P, dp, I64, dp]
         L.tecno_compute_globals.argtypes = [P, C.POINTER(Globals)]
         L.tecno_vorticity.argtypes = [P, dp, I64]
@@ -312,6 +313,12 @@ class Solver:
     def audit(self):
         a = Audit()
         _check(lib().tecno_audit_state(self.h, C.byref(a)))
+        return a
+
+    def audit_noalloc(self):
+        """tecno_audit_state_noalloc: the audit with no allocation"""
+        a = Audit()
+        _check(lib().tecno_audit_state_noalloc(self.h, C.byref(a)))
         return a
 
     def globals(self):

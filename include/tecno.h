@@ -218,6 +218,19 @@ typedef struct tecno_audit {
 } tecno_audit;
 int tecno_audit_state(tecno_ctx* ctx, tecno_audit* out);
 
+/* tecno_audit_state without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * stepping loops that audit every step, harnesses that forbid allocation;
+ * tecno_audit_state may allocate its staging buffer on first use).  Same
+ * kernels and the same summation order, so the same results bit for bit.
+ * The partial sums live in a fixed 73.8 KB device array that every GPU the
+ * library runs on holds until the process ends; its size does not depend on
+ * the grid (the audit's 1024 blocks stride over the cells), so no grid is
+ * rejected.  Calls that overlap on one GPU share that array and race; run
+ * them one at a time per GPU.  The call makes the context's GPU current for
+ * its duration. */
+int tecno_audit_state_noalloc(tecno_ctx* ctx, tecno_audit* out);
+
 /* L1 error of the primitive state against exact values (compute_error,
  * enflo/src/compute_errors.cc:9-34): out[v] = sum |exact-prim| * dV.
  * Sums here and in tecno_audit_state are the reference's sums: a cell of

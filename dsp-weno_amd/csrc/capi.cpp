@@ -124,6 +124,7 @@ static void fill_phys(const tecno_ctx* c, Phys<T>& P) {
   P.fix_zmom = g.fix_zmom;
   P.sign_fallback = g.sign_fallback;
   P.fix_edges = g.fix_edges;
+  P.fix_visc = g.fix_visc;
   P.mlp_mode = g.mlp_dtype;
   P.gamma = T(g.gamma);
   P.R = T(g.gas_const);

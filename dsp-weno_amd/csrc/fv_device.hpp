@@ -399,7 +399,7 @@ __device__ __forceinline__ void muscl(const T* st, int s, int kind, T& l, T& r) 
 template <typename T>
 struct Phys {
   int system, model, flux, recon, corr, eno_k;
-  int ec1_fix, fix_zmom, sign_fallback, arch5, mlp_mode, fix_edges;
+  int ec1_fix, fix_zmom, sign_fallback, arch5, mlp_mode, fix_edges, fix_visc;
   T gamma, R, gm1, inv_gm1;
   T vel[3];      // linear advection velocity
   T area[3];     // face areas (param.in spacings)

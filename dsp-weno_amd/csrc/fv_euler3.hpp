@@ -427,12 +427,25 @@ struct Euler3Ph {
       Y[1] += c2 * e4 * (-Tz[3] + e3 * Tz[4]);
       Y[2] += mu * e4 * (-Tz[2] + e2 * Tz[4]);
       Y[3] += e4 * (mu * e3 * Tz[2] + c2 * e2 * Tz[3] - (mu + c2) * e2 * e3 * Tz[4]);
+      if (P.fix_visc) {  // K21 V_x (vflux_ES.cc:103-108), kept
+        Y[0] += mu * e4 * (-Tx[2] + e2 * Tx[4]);
+        Y[1] += c2 * e4 * (-Tx[1] + e1 * Tx[4]);
+        Y[3] += e4 * (c2 * e2 * Tx[1] + mu * e1 * Tx[2] - (mu + c2) * e1 * e2 * Tx[4]);
+      }
       // z flux: K31 and K32 parts overwritten by the K33 assignment
       Z[0] = mu * e4 * (-Tz[1] + e1 * Tz[4]);
       Z[1] = mu * e4 * (-Tz[2] + e2 * Tz[4]);
       Z[2] = c1 * e4 * (-Tz[3] + e3 * Tz[4]);
       Z[3] = e4 * (mu * e1 * Tz[1] + mu * e2 * Tz[2] + c1 * e3 * Tz[3] +
                    (-c1 * e3 * e3 - mu * (e1 * e1 + e2 * e2) + c3 * e4) * Tz[4]);
+      if (P.fix_visc) {  // K31 V_x and K32 V_y (vflux_ES.cc:127-138), kept
+        Z[0] += mu * e4 * (-Tx[3] + e3 * Tx[4]);
+        Z[2] += c2 * e4 * (-Tx[1] + e1 * Tx[4]);
+        Z[3] += e4 * (c2 * e3 * Tx[1] + mu * e1 * Tx[3] - (mu + c2) * e1 * e3 * Tx[4]);
+        Z[1] += mu * e4 * (-Ty[3] + e3 * Ty[4]);
+        Z[2] += c2 * e4 * (-Ty[2] + e2 * Ty[4]);
+        Z[3] += e4 * (c2 * e3 * Ty[2] + mu * e2 * Ty[3] - (mu + c2) * e2 * e3 * Ty[4]);
+      }
     } else {
       // nodal_vlux_KE vflux_KE.cc:15-96 with nodal_avg_and_vel_derivatives
       const T fx = T(1) / (T(4) * dx), fy = T(1) / (T(4) * dy), fz = T(1) / (T(4) * dz);

@@ -67,7 +67,7 @@ class Cfg(C.Structure):
         ("mu_model", C.c_int32), ("mu_ref", C.c_double), ("T_ref", C.c_double),
         ("T_0", C.c_double), ("omega", C.c_double), ("ec1_fix", C.c_int32),
         ("fix_zmom", C.c_int32), ("sign_fallback", C.c_int32), ("fix_edges", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("fix_visc", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -78,7 +78,7 @@ def make_cfg(*, system="euler", flux="kepes_tecno4_roe", recon="sp_weno", n=(32,
              lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dims=2, bc="periodic", cfl=0.5,
              vel=(1.0, 0.0, 0.0), model=None, precision=FP64, mlp_dtype=MLP_FP64,
              gamma=1.4, gas_const=1.0, ec1_fix=0, fix_zmom=0, sign_fallback=0, fix_edges=0,
-             time_scheme=TS_SSPRK3, viscosity=("constant", 0.0), prandtl=0.72):
+             time_scheme=TS_SSPRK3, viscosity=("constant", 0.0), prandtl=0.72, fix_visc=0):
     """viscosity: ("constant", mu_ref) | ("sutherland", mu_ref, T_ref, T_0) |
     ("power", mu_ref, T_ref, omega), as param.in's material/viscosity block"""
     c = Cfg()
@@ -120,6 +120,7 @@ def make_cfg(*, system="euler", flux="kepes_tecno4_roe", recon="sp_weno", n=(32,
     c.fix_zmom = int(fix_zmom)
     c.sign_fallback = int(sign_fallback)
     c.fix_edges = int(fix_edges)
+    c.fix_visc = int(fix_visc)
     return c
 
 

@@ -127,7 +127,13 @@ typedef struct tecno_cfg {
                             the reference's boundary loops do
                             (fv.cc:821-1156); 1 once.  Nodes on a neumann
                             or wall boundary count once either way.        */
-  int32_t reserved[7];
+  int32_t fix_visc;      /* ns_es: 0 drops the K21 V_x term of the y viscous
+                            flux and the K31 V_x, K32 V_y terms of the z
+                            flux, which the reference's nodal_vlux_ES adds
+                            and then overwrites (vflux_ES.cc:103-145), so
+                            its viscous matrix is not symmetric; 1 keeps
+                            them (K symmetric positive semi-definite)      */
+  int32_t reserved[6];
 } tecno_cfg;
 
 typedef struct tecno_ctx tecno_ctx;

@@ -1301,34 +1301,15 @@ static void euler_residual(tor_solver* s) {
 /* ---- Navier-Stokes viscous terms (periodic grids) --------------------- */
 typedef struct { double m[5]; } flux5; /* mass, mom x/y/z, energy */
 
-/* nodal_vlux_ES vflux_ES.cc:15-151, including its ordering: the K21 and K31,
- * K32 contributions to the y and z fluxes are overwritten by the K22 / K33
- * assignments, so y = K22 V_y + K23 V_z and z = K33 V_z. */
-static void node_flux_es(const tor_solver* s, int i, int j, int k, flux5* X, flux5* Y, flux5* Z) {
-  const tecno_cfg* c = &s->c;
-  const double dx = s->g.h[0], dy = s->g.h[1], dz = s->g.h[2];
-  /* cells: index (a,b,c) in {0,1}^3 -> V[pidx(i+a, j+b, k+c)] */
-#define VV(a, b, cc) (&s->V[pidx(s, i + (a), j + (b), k + (cc))])
-  const ent_t* E[2][2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int q = 0; q < 2; ++q) E[a][b][q] = VV(a, b, q);
-#undef VV
-  double avg[5], Tx[5], Ty[5], Tz[5];
-  const double f1 = 1.0 / 8.0, fx = 1.0 / 4.0 / dx, fy = 1.0 / 4.0 / dy, fz = 1.0 / 4.0 / dz;
-  for (int comp = 0; comp < 5; ++comp) {
-#define C(e) ((comp == 0) ? (e)->f : (comp == 4) ? (e)->l : (e)->v[comp - 1])
-    /* ent_avg: (111)(011)(101)(110) then (001)(010)(100)(000) */
-    avg[comp] = C(E[1][1][1]) * f1 + C(E[0][1][1]) * f1 + C(E[1][0][1]) * f1 + C(E[1][1][0]) * f1;
-    avg[comp] += C(E[0][0][1]) * f1 + C(E[0][1][0]) * f1 + C(E[1][0][0]) * f1 + C(E[0][0][0]) * f1;
-    Tx[comp] = C(E[1][1][1]) * fx + C(E[1][1][0]) * fx + C(E[1][0][1]) * fx + C(E[1][0][0]) * fx;
-    Tx[comp] += C(E[0][1][1]) * -fx + C(E[0][1][0]) * -fx + C(E[0][0][1]) * -fx + C(E[0][0][0]) * -fx;
-    Ty[comp] = C(E[1][1][1]) * fy + C(E[1][1][0]) * fy + C(E[0][1][1]) * fy + C(E[0][1][0]) * fy;
-    Ty[comp] += C(E[1][0][1]) * -fy + C(E[1][0][0]) * -fy + C(E[0][0][1]) * -fy + C(E[0][0][0]) * -fy;
-    Tz[comp] = C(E[1][1][1]) * fz + C(E[1][0][1]) * fz + C(E[0][1][1]) * fz + C(E[0][0][1]) * fz;
-    Tz[comp] += C(E[1][1][0]) * -fz + C(E[1][0][0]) * -fz + C(E[0][1][0]) * -fz + C(E[0][0][0]) * -fz;
-#undef C
-  }
+/* the ES viscous node fluxes from the node average and differences of V
+ * (vflux_ES.cc:74-145), before the face areas: X, Y, Z = momentum x, y, z
+ * and energy of the x, y and z fluxes (m[0], the mass flux, is 0).  The
+ * reference adds the K21 V_x part of Y and the K31 V_x, K32 V_y parts of Z
+ * and then overwrites them with the K22 / K33 assignments (vflux_ES.cc:
+ * 103-145), so y = K22 V_y + K23 V_z and z = K33 V_z; cfg->fix_visc keeps
+ * them (y = K21 V_x + K22 V_y + K23 V_z, z = K31 V_x + K32 V_y + K33 V_z). */
+static void es_flux_core(const tecno_cfg* c, const double avg[5], const double Tx[5],
+                         const double Ty[5], const double Tz[5], flux5* X, flux5* Y, flux5* Z) {
   const double T_avg = -1 / avg[4] / c->gas_const;
   const double mu = viscosity(c, T_avg);
   const double Cp = c->gamma * c->gas_const / (c->gamma - 1.0); /* parameter.cc:286 */
@@ -1361,6 +1342,11 @@ static void node_flux_es(const tor_solver* s, int i, int j, int k, flux5* X, flu
   Y->m[2] += c2 * e4 * (-Tz[3] + e3 * Tz[4]);
   Y->m[3] += mu * e4 * (-Tz[2] + e2 * Tz[4]);
   Y->m[4] += e4 * (mu * e3 * Tz[2] + c2 * e2 * Tz[3] - (mu + c2) * e2 * e3 * Tz[4]);
+  if (c->fix_visc) { /* K21 V_x */
+    Y->m[1] += mu * e4 * (-Tx[2] + e2 * Tx[4]);
+    Y->m[2] += c2 * e4 * (-Tx[1] + e1 * Tx[4]);
+    Y->m[4] += e4 * (c2 * e2 * Tx[1] + mu * e1 * Tx[2] - (mu + c2) * e1 * e2 * Tx[4]);
+  }
   /* Z: K33 V_z (assigned) */
   Z->m[0] = 0.0;
   Z->m[1] = mu * e4 * (-Tz[1] + e1 * Tz[4]);
@@ -1368,6 +1354,55 @@ static void node_flux_es(const tor_solver* s, int i, int j, int k, flux5* X, flu
   Z->m[3] = c1 * e4 * (-Tz[3] + e3 * Tz[4]);
   Z->m[4] = e4 * (mu * e1 * Tz[1] + mu * e2 * Tz[2] + c1 * e3 * Tz[3] +
                   (-c1 * e3 * e3 - mu * (e1 * e1 + e2 * e2) + c3 * e4) * Tz[4]);
+  if (c->fix_visc) { /* K31 V_x, K32 V_y */
+    Z->m[1] += mu * e4 * (-Tx[3] + e3 * Tx[4]);
+    Z->m[3] += c2 * e4 * (-Tx[1] + e1 * Tx[4]);
+    Z->m[4] += e4 * (c2 * e3 * Tx[1] + mu * e1 * Tx[3] - (mu + c2) * e1 * e3 * Tx[4]);
+    Z->m[2] += mu * e4 * (-Ty[3] + e3 * Ty[4]);
+    Z->m[3] += c2 * e4 * (-Ty[2] + e2 * Ty[4]);
+    Z->m[4] += e4 * (c2 * e3 * Ty[2] + mu * e2 * Ty[3] - (mu + c2) * e2 * e3 * Ty[4]);
+  }
+}
+
+void tor_es_node_flux(const tecno_cfg* cfg, const double avg[5], const double Tx[5],
+                      const double Ty[5], const double Tz[5], double out[12]) {
+  flux5 X, Y, Z;
+  es_flux_core(cfg, avg, Tx, Ty, Tz, &X, &Y, &Z);
+  for (int v = 0; v < 4; ++v) {
+    out[v] = X.m[1 + v];
+    out[4 + v] = Y.m[1 + v];
+    out[8 + v] = Z.m[1 + v];
+  }
+}
+
+/* nodal_vlux_ES vflux_ES.cc:15-151: node average and differences of the
+ * entropy variables of the 8 cells, es_flux_core, face areas */
+static void node_flux_es(const tor_solver* s, int i, int j, int k, flux5* X, flux5* Y, flux5* Z) {
+  const tecno_cfg* c = &s->c;
+  const double dx = s->g.h[0], dy = s->g.h[1], dz = s->g.h[2];
+  /* cells: index (a,b,c) in {0,1}^3 -> V[pidx(i+a, j+b, k+c)] */
+#define VV(a, b, cc) (&s->V[pidx(s, i + (a), j + (b), k + (cc))])
+  const ent_t* E[2][2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int q = 0; q < 2; ++q) E[a][b][q] = VV(a, b, q);
+#undef VV
+  double avg[5], Tx[5], Ty[5], Tz[5];
+  const double f1 = 1.0 / 8.0, fx = 1.0 / 4.0 / dx, fy = 1.0 / 4.0 / dy, fz = 1.0 / 4.0 / dz;
+  for (int comp = 0; comp < 5; ++comp) {
+#define C(e) ((comp == 0) ? (e)->f : (comp == 4) ? (e)->l : (e)->v[comp - 1])
+    /* ent_avg: (111)(011)(101)(110) then (001)(010)(100)(000) */
+    avg[comp] = C(E[1][1][1]) * f1 + C(E[0][1][1]) * f1 + C(E[1][0][1]) * f1 + C(E[1][1][0]) * f1;
+    avg[comp] += C(E[0][0][1]) * f1 + C(E[0][1][0]) * f1 + C(E[1][0][0]) * f1 + C(E[0][0][0]) * f1;
+    Tx[comp] = C(E[1][1][1]) * fx + C(E[1][1][0]) * fx + C(E[1][0][1]) * fx + C(E[1][0][0]) * fx;
+    Tx[comp] += C(E[0][1][1]) * -fx + C(E[0][1][0]) * -fx + C(E[0][0][1]) * -fx + C(E[0][0][0]) * -fx;
+    Ty[comp] = C(E[1][1][1]) * fy + C(E[1][1][0]) * fy + C(E[0][1][1]) * fy + C(E[0][1][0]) * fy;
+    Ty[comp] += C(E[1][0][1]) * -fy + C(E[1][0][0]) * -fy + C(E[0][0][1]) * -fy + C(E[0][0][0]) * -fy;
+    Tz[comp] = C(E[1][1][1]) * fz + C(E[1][0][1]) * fz + C(E[0][1][1]) * fz + C(E[0][0][1]) * fz;
+    Tz[comp] += C(E[1][1][0]) * -fz + C(E[1][0][0]) * -fz + C(E[0][1][0]) * -fz + C(E[0][0][0]) * -fz;
+#undef C
+  }
+  es_flux_core(c, avg, Tx, Ty, Tz, X, Y, Z);
   const double ax = -(dy * dz), ay = -(dx * dz), az = -(dx * dy);
   for (int v = 0; v < 5; ++v) {
     X->m[v] *= ax;

@@ -74,6 +74,12 @@ typedef struct tor_solver tor_solver;
  * exactly as enflo/enfloS do (a "2D" run is Nz = 2 with z faces).
  * faithful = 0: only the first cfg->dims directions are active; inactive
  * directions hold one cell and keep the param.in spacing. */
+/* the ES viscous node fluxes (momentum x, y, z and energy of the x, y and z
+ * fluxes, before the face areas) from the node average of V and its
+ * differences along x, y, z (vflux_ES.cc:74-145; cfg->fix_visc keeps the
+ * K21, K31, K32 parts the reference overwrites) */
+void tor_es_node_flux(const tecno_cfg* cfg, const double avg[5], const double Tx[5],
+                      const double Ty[5], const double Tz[5], double out[12]);
 tor_solver* tor_create(const tecno_cfg* cfg, const tor_net* net, int faithful);
 void tor_destroy(tor_solver* s);
 int64_t tor_num_cells(const tor_solver* s);

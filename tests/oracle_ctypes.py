@@ -53,6 +53,7 @@ def lib():
         L.tor_advance.argtypes = [P, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         L.tor_set_threads.argtypes = [C.c_int]
         L.tor_audit.argtypes = [P, dp]
+        L.tor_es_node_flux.argtypes = [C.POINTER(Cfg), dp, dp, dp, dp, dp]
         L.tor_train_rows.argtypes = [dp, dp, C.c_int64, dp]
         L.tor_train_loss_grad.argtypes = [C.POINTER(Net), dp, C.c_int64, C.c_int, C.POINTER(Net)]
         L.tor_train_loss_grad.restype = C.c_double
@@ -213,6 +214,15 @@ class OracleSolver:
         out = np.zeros(8)
         lib().tor_audit(self.h, _dp(out))
         return out
+
+
+def es_node_flux(cfg, avg, Tx, Ty, Tz):
+    """ES viscous node fluxes before the face areas: (3 fluxes, 4 components:
+    momentum x, y, z, energy)"""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (avg, Tx, Ty, Tz)]
+    out = np.zeros(12)
+    lib().tor_es_node_flux(C.byref(cfg), *[_dp(v) for v in a], _dp(out))
+    return out.reshape(3, 4)
 
 
 # ---------------------------------------------------------------- trainer

@@ -616,8 +616,14 @@ struct Euler2Mixed {
       // features, vertices and activations are live at a time
 #pragma unroll 1
       for (int i = 0; i < 4; ++i) {
-        const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
         const float d1 = sel4(i, dZ[0][1], dZ[1][1], dZ[2][1], dZ[3][1]);
+        const bool skip = !(fabsf(d1) >= 1e-15f);
+        // a component whose centre jump is zero on every lane of the wave
+        // needs no network: its reconstructed jump is 0 (reconstruct.cc:152)
+        // and jmp[i] already holds it.  Uniform regions of a flow (a shear
+        // layer's free streams) have whole waves of such faces.
+        if (__all(skip)) continue;
+        const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
         const float d2 = sel4(i, dZ[0][2], dZ[1][2], dZ[2][2], dZ[3][2]);
         const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
         const float z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;
@@ -628,7 +634,6 @@ struct Euler2Mixed {
         // finite, and the partner lane of its matrix column would see them.
         // Otherwise the features are finite for a finite state (tanh of an
         // overflowing ratio saturates to +-1).
-        const bool skip = !(fabsf(d1) >= 1e-15f);
 #pragma unroll
         for (int k = 0; k < 5; ++k) st[k] = skip ? 0.0f : st[k];
         net.eval(st, lg);
@@ -644,8 +649,11 @@ struct Euler2Mixed {
       // FP32 network on the vector units, components one after the other
 #pragma unroll 1
       for (int i = 0; i < 4; ++i) {
-        const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
         const float d1 = sel4(i, dZ[0][1], dZ[1][1], dZ[2][1], dZ[3][1]);
+#ifndef TECNO_HOST_BUILD
+        if (__all(!(fabsf(d1) >= 1e-15f))) continue;  // zero jump on every lane
+#endif
+        const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
         const float d2 = sel4(i, dZ[0][2], dZ[1][2], dZ[2][2], dZ[3][2]);
         const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
         const float z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;

@@ -133,3 +133,100 @@ def test_trainer_data_file_round_trip(tmp_path):
         for r, c, t in zip(rows, u4, tg):
             f.write(",".join(f"{x:.17g}" for x in [*c, *r[0:15], *t]) + "\n")
     assert np.array_equal(R.read_trainer_data(str(p21)), rows)
+
+
+def _smooth_prim(x, y):
+    tp = 2 * np.pi
+    return (1.0 + 0.2 * np.sin(tp * x) * np.cos(tp * y), 0.3 * np.sin(tp * (x + y)),
+            0.2 * np.cos(tp * x) - 0.1 * np.sin(tp * y), 1.0 + 0.15 * np.cos(tp * y) * np.sin(tp * x))
+
+
+def _fine_point_state(nf):
+    c = (np.arange(nf) + 0.5) / nf
+    X, Y = np.meshgrid(c, c)
+    return _smooth_prim(X, Y)
+
+
+def _frame(q, axis):
+    """(rho, n, t, p) or V in the face frame of the axis"""
+    return (q[0], q[1 + axis], q[2 - axis], q[3])
+
+
+def test_fine_harvest_targets_converge_to_face_means():
+    """Targets of the fine-run harvest against the exact mean, along each
+    coarse face, of the face-frame entropy variables projected on that
+    face's eigenvectors (5-point Gauss quadrature of the smooth field):
+    with the coarse mesh fixed and the fine mesh refined 2x, 4x, 8x the
+    largest error falls at second order (the fine state holds point values,
+    so the midpoint rule is the leading error); a stencil taken one fine
+    column off, or rows of the wrong coarse row, stays at first order or
+    does not converge at all."""
+    import oracle_ctypes as orc
+    n = 8
+    gx, gw = np.polynomial.legendre.leggauss(5)
+    err = []
+    for f in (2, 4, 8):
+        u4, tg, faces = R.harvest_euler_2d_fine(_fine_point_state(n * f), f, orc.sp_weno,
+                                                with_faces=True)
+        coarse = R.restrict(_fine_point_state(n * f), f)
+        worst = 0.0
+        for (axis, w, J, I), t in zip(faces, tg):
+            q = _frame([c if axis == 0 else c.T for c in coarse], axis)
+            Rm = R.face_eigvecs(*[np.array([a[J, I]]) for a in q],
+                                *[np.array([a[J, I + 1]]) for a in q])[w, :, 0]
+            s = (I + 1) / n
+            tq = (J + 0.5 * (gx + 1)) / n
+            xs, ys = (np.full(5, s), tq) if axis == 0 else (tq, np.full(5, s))
+            V = _frame(R.entropy_vars(*_smooth_prim(xs, ys)), axis)
+            exact = 0.5 * np.dot(gw, sum(Rm[r] * V[r] for r in range(4)))
+            worst = max(worst, abs(t[0] - exact), abs(t[1] - exact))
+        err.append(worst)
+        assert len(u4) == len(faces) == 2 * 4 * n * (n - 3)
+    assert err[0] / err[1] > 3.0 and err[1] / err[2] > 3.0, err
+
+
+def test_fine_harvest_stencils_are_the_coarse_harvest():
+    """The stencils of the fine-run harvest are the characteristic stencils
+    of the restricted state (those of harvest_euler_2d at the same faces),
+    and transposing the state (x <-> y, u <-> v) swaps the two axes' rows."""
+    import oracle_ctypes as orc
+    n, f = 12, 4
+    prim = _fine_point_state(n * f)
+    u4, tg, faces = R.harvest_euler_2d_fine(prim, f, orc.sp_weno, with_faces=True)
+    coarse = R.restrict(prim, f)
+    full, _ = R.harvest_euler_2d(coarse, lambda axis: coarse, drop_rel=-1.0)
+    full = full.reshape(2, 4, n, n, 4)  # axis, wave, y, x
+    for (axis, w, J, I), st in zip(faces, u4):
+        ref = full[0, w, J, I] if axis == 0 else full[1, w, I, J]
+        assert np.abs(st - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    tprim = (prim[0].T, prim[2].T, prim[1].T, prim[3].T)
+    u4t, tgt, facest = R.harvest_euler_2d_fine(tprim, f, orc.sp_weno, with_faces=True)
+    key = {tuple(fc): k for k, fc in enumerate(faces)}
+    assert len(facest) == len(faces)
+    for k, (axis, w, J, I) in enumerate(facest):
+        j = key[(1 - axis, w, J, I)]
+        assert np.abs(u4t[k] - u4[j]).max() <= 1e-12 and np.abs(tgt[k] - tg[j]).max() <= 1e-12
+
+
+def test_fine_harvest_keeps_both_sides_of_a_face_discontinuity():
+    """A contact discontinuity lying on a coarse face (density jump, equal
+    velocity and pressure): the entropy-wave stencil there has a step, and
+    its left and right targets are the fine reconstructions' one-sided
+    values, which for a step of constant states are the two states
+    themselves to the SP-WENO's smoothness weighting."""
+    import oracle_ctypes as orc
+    n, f = 8, 4
+    c = (np.arange(n * f) + 0.5) / (n * f)
+    X, Y = np.meshgrid(c, c)
+    rho = np.where(X < 0.5, 1.0, 0.25)
+    prim = (rho, 0.1 + 0 * X, 0 * X, 1.0 + 0 * X)
+    u4, tg, faces = R.harvest_euler_2d_fine(prim, f, orc.sp_weno, with_faces=True)
+    m = (faces[:, 0] == 0) & (faces[:, 3] == n // 2 - 1)
+    assert m.any()
+    st, t = u4[m], tg[m]
+    assert np.allclose(st[:, 0], st[:, 1]) and np.allclose(st[:, 2], st[:, 3])
+    jump = np.abs(st[:, 2] - st[:, 1])
+    assert np.all(np.abs(t[:, 0] - st[:, 1]) <= 1e-3 * jump)
+    assert np.all(np.abs(t[:, 1] - st[:, 2]) <= 1e-3 * jump)
+    # no stencil away from the discontinuity: constant states have no jump
+    assert np.all(np.abs(faces[:, 3] - (n // 2 - 1)) <= 1)

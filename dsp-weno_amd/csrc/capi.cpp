@@ -245,7 +245,11 @@ static int upload_net(tecno_ctx* c) {
   return 0;
 }
 
-static bool fp64(const tecno_ctx* c) { return c->cfg.precision == TECNO_FP64; }
+// the FP64 kernel family: FP64, and scalar FP32 arithmetic on an FP64 state
+// (whose stage kernel differs in the face only, L.scalar_mixed)
+static bool fp64(const tecno_ctx* c) {
+  return c->cfg.precision == TECNO_FP64 || c->cfg.precision == TECNO_FP32_STATE64;
+}
 
 // cells the reference stores per held cell: the param.in cell counts of the
 // collapsed axes (a "2D" run holds Nx x Ny x Nz_cell = 2 cells in the
@@ -339,8 +343,11 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
     return fail(TECNO_EINVAL, "eno k: k must be 2 or 3");
   if (g.time_scheme < TECNO_TS_RK1 || g.time_scheme > TECNO_TS_RK4)
     return fail(TECNO_EINVAL, "unknown time scheme");
-  if (g.precision != TECNO_FP64 && g.precision != TECNO_FP32)
-    return fail(TECNO_EINVAL, "precision must be FP64 or FP32");
+  if (g.precision != TECNO_FP64 && g.precision != TECNO_FP32 && g.precision != TECNO_FP32_STATE64)
+    return fail(TECNO_EINVAL, "precision must be FP64, FP32 or FP32_STATE64");
+  if (g.precision == TECNO_FP32_STATE64 &&
+      (g.system != TECNO_SYS_SCALAR || g.recon != TECNO_REC_SP_WENO))
+    return fail(TECNO_EINVAL, "FP32_STATE64: scalar problems with sp_weno, sp_wenoc or corr2 only");
   for (int i = 0; i < 6; ++i)
     if (g.bc[i] == TECNO_BC_DIRICHLET || g.bc[i] < 0 || g.bc[i] > 3)
       return fail(TECNO_EINVAL, "boundary type must be periodic, neumann or wall");
@@ -413,6 +420,7 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   c->L.dims = g.dims;
   c->L.G = c->G;
   c->L.fast_flux = g.flux == TECNO_FLUX_KEPES_TECNO4_ROE && !g.ec1_fix ? FV_KEPES4_ROE : FV_GENERIC;
+  c->L.scalar_mixed = g.precision == TECNO_FP32_STATE64 ? 1 : 0;
   c->L.mlp_mode = g.mlp_dtype == TECNO_MLP_BF16 ? MM_BF16
                   : g.mlp_dtype == TECNO_MLP_FP16 ? MM_FP16
                                                    : MM_SIMT;
@@ -1317,7 +1325,7 @@ int tecno_init_net(tecno_ctx* c, int nl, const int32_t* widths, uint64_t seed) {
 
 namespace {
 struct EnsGroupH {
-  bool fp32 = false;
+  int prec = ENS_FP64;  // ENS_* of the members (their cfg.precision)
   int dims = 1;
   int variant = 0;
   std::vector<int> members;
@@ -1367,7 +1375,7 @@ static int ens_build_work(tecno_ens* e, EnsGroupH& G) {
   G.nwork = (int)w.size();
   HIPCHK(hipMalloc(&G.dwork, w.size() * sizeof(EnsWork)));
   HIPCHK(hipMemcpy(G.dwork, w.data(), w.size() * sizeof(EnsWork), hipMemcpyHostToDevice));
-  const int resident = ens_grid(G.fp32, G.dims, G.variant, e->n_cu);
+  const int resident = ens_grid(G.prec, G.dims, G.variant, e->n_cu);
   G.grid = std::max(1, std::min(G.nwork, resident));
   ens_drop_graph(e);
   return 0;
@@ -1398,7 +1406,7 @@ static int ens_upload(tecno_ens* e) {
   if (!e->dctl) HIPCHK(hipMalloc(&e->dctl, hc.size() * sizeof(EnsCtl)));
   HIPCHK(hipMemcpy(e->dctl, hc.data(), hc.size() * sizeof(EnsCtl), hipMemcpyHostToDevice));
   for (auto& G : e->g) {
-    const int rc = G.fp32 ? ens_upload_probs_t<float>(e, G, &tecno_ctx::P32)
+    const int rc = G.prec == ENS_FP32 ? ens_upload_probs_t<float>(e, G, &tecno_ctx::P32)
                           : ens_upload_probs_t<double>(e, G, &tecno_ctx::P64);
     if (rc) return rc;
   }
@@ -1413,7 +1421,7 @@ static hipError_t ens_step(tecno_ens* e, int only_group) {
     for (int gi = 0; gi < (int)e->g.size() && r == hipSuccess; ++gi) {
       if (only_group >= 0 && gi != only_group) continue;
       const EnsGroupH& G = e->g[gi];
-      r = ens_stage(e->s, G.fp32, G.dims, G.variant, G.dprobs, G.dwork, G.nwork, st, G.grid);
+      r = ens_stage(e->s, G.prec, G.dims, G.variant, G.dprobs, G.dwork, G.nwork, st, G.grid);
     }
   if (r == hipSuccess) r = ens_control(e->s, e->dctl, n, 1);
   return r;
@@ -1461,15 +1469,17 @@ int tecno_ens_create(const tecno_cfg* cfgs, int n, tecno_ens** out) {
     if (int rc = tecno_create(&cfgs[i], &c)) return bail(rc);
     e->m.push_back(c);
     if (c->G != 2) return bail(fail(TECNO_EINVAL, "ensembles use reconstructions with 2 ghost cells"));
-    const bool fp32 = cfgs[i].precision == TECNO_FP32;
+    const int prec = cfgs[i].precision == TECNO_FP32           ? ENS_FP32
+                     : cfgs[i].precision == TECNO_FP32_STATE64 ? ENS_FP32_STATE64
+                                                               : ENS_FP64;
     const int dims = cfgs[i].dims;
     int gi = -1;
     for (int q = 0; q < (int)e->g.size(); ++q)
-      if (e->g[q].fp32 == fp32 && e->g[q].dims == dims) gi = q;
+      if (e->g[q].prec == prec && e->g[q].dims == dims) gi = q;
     if (gi < 0) {
       e->g.emplace_back();
       gi = (int)e->g.size() - 1;
-      e->g[gi].fp32 = fp32;
+      e->g[gi].prec = prec;
       e->g[gi].dims = dims;
     }
     e->g[gi].members.push_back(i);
@@ -1517,7 +1527,9 @@ int tecno_ens_group_info(const tecno_ens* e, int gi, tecno_ens_group* out) {
   if (!e || !out || gi < 0 || gi >= (int)e->g.size()) return fail(TECNO_EINVAL, "bad group");
   const EnsGroupH& G = e->g[gi];
   std::memset(out, 0, sizeof(*out));
-  out->precision = G.fp32 ? TECNO_FP32 : TECNO_FP64;
+  out->precision = G.prec == ENS_FP32           ? TECNO_FP32
+                  : G.prec == ENS_FP32_STATE64 ? TECNO_FP32_STATE64
+                                               : TECNO_FP64;
   out->dims = G.dims;
   out->n_members = (int)G.members.size();
   out->n_tiles = G.nwork;

@@ -4,13 +4,16 @@
 
 namespace tecno {
 
-template hipError_t ens_launch<double>(hipStream_t, int, int, const void*, const EnsWork*, int, int,
-                                       int);
-template int ens_grid_of<double>(int, int, int);
+template hipError_t ens_launch<ScalarPh<double, 2>>(hipStream_t, int, int, const void*,
+                                                     const EnsWork*, int, int, int);
+template int ens_grid_of<ScalarPh<double, 2>>(int, int, int);
+template hipError_t ens_launch<ScalarMixedPh<2>>(hipStream_t, int, int, const void*,
+                                                 const EnsWork*, int, int, int);
+template int ens_grid_of<ScalarMixedPh<2>>(int, int, int);
 // the FP32 kernels are built in ens_fp32.hip
-extern template hipError_t ens_launch<float>(hipStream_t, int, int, const void*, const EnsWork*,
-                                             int, int, int);
-extern template int ens_grid_of<float>(int, int, int);
+extern template hipError_t ens_launch<ScalarPh<float, 2>>(hipStream_t, int, int, const void*,
+                                                          const EnsWork*, int, int, int);
+extern template int ens_grid_of<ScalarPh<float, 2>>(int, int, int);
 
 __global__ void k_ens_control(const EnsCtl* cs, int n, int finish) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -63,15 +66,25 @@ hipError_t ens_count_active(hipStream_t s, const EnsCtl* c, int n, int* out) {
   return hipGetLastError();
 }
 
-hipError_t ens_stage(hipStream_t s, bool fp32, int dims, int variant, const void* probs,
+hipError_t ens_stage(hipStream_t s, int prec, int dims, int variant, const void* probs,
                      const EnsWork* work, int nwork, int stage, int grid) {
   if (nwork <= 0) return hipSuccess;
-  return fp32 ? ens_launch<float>(s, dims, variant, probs, work, nwork, stage, grid)
-              : ens_launch<double>(s, dims, variant, probs, work, nwork, stage, grid);
+  switch (prec) {
+    case ENS_FP32:
+      return ens_launch<ScalarPh<float, 2>>(s, dims, variant, probs, work, nwork, stage, grid);
+    case ENS_FP32_STATE64:
+      return ens_launch<ScalarMixedPh<2>>(s, dims, variant, probs, work, nwork, stage, grid);
+    default:
+      return ens_launch<ScalarPh<double, 2>>(s, dims, variant, probs, work, nwork, stage, grid);
+  }
 }
 
-int ens_grid(bool fp32, int dims, int variant, int n_cu) {
-  return fp32 ? ens_grid_of<float>(dims, variant, n_cu) : ens_grid_of<double>(dims, variant, n_cu);
+int ens_grid(int prec, int dims, int variant, int n_cu) {
+  switch (prec) {
+    case ENS_FP32: return ens_grid_of<ScalarPh<float, 2>>(dims, variant, n_cu);
+    case ENS_FP32_STATE64: return ens_grid_of<ScalarMixedPh<2>>(dims, variant, n_cu);
+    default: return ens_grid_of<ScalarPh<double, 2>>(dims, variant, n_cu);
+  }
 }
 
 }  // namespace tecno

@@ -4,6 +4,7 @@
 // as the FP32 ones), so that ensemble members are bitwise single runs.
 #pragma once
 #include "ensemble.hpp"
+#include "fv_mixed.hpp"
 
 namespace tecno {
 
@@ -58,68 +59,62 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(const EnsProb<typename P
   }
 }
 
-template <typename T, int DIMS, int V>
-struct EnsK {
-  static constexpr int TX = ens_tx(DIMS, V), TY = ens_ty(DIMS, V);
-};
-
-template <typename T, int DIMS, int V>
+template <class PH, int DIMS, int V>
 hipError_t ens_launch_v(hipStream_t s, const void* probs, const EnsWork* work, int nwork, int stage,
-                    int grid) {
-  constexpr int TX = EnsK<T, DIMS, V>::TX, TY = EnsK<T, DIMS, V>::TY;
-  hipLaunchKernelGGL((ens_stage_kernel<ScalarPh<T, 2>, DIMS, TX, TY, 2>), dim3(grid), dim3(256), 0, s,
-                     (const EnsProb<T>*)probs, work, nwork, stage);
+                        int grid) {
+  constexpr int TX = ens_tx(DIMS, V), TY = ens_ty(DIMS, V);
+  hipLaunchKernelGGL((ens_stage_kernel<PH, DIMS, TX, TY, 2>), dim3(grid), dim3(256), 0, s,
+                     (const EnsProb<typename PH::TP>*)probs, work, nwork, stage);
   return hipGetLastError();
 }
 
-template <typename T, int DIMS, int V>
+template <class PH, int DIMS, int V>
 int ens_grid_v(int n_cu) {
   int nb = 0;
-  constexpr int TX = EnsK<T, DIMS, V>::TX, TY = EnsK<T, DIMS, V>::TY;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &nb, ens_stage_kernel<ScalarPh<T, 2>, DIMS, TX, TY, 2>, 256, 0) !=
-          hipSuccess ||
+  constexpr int TX = ens_tx(DIMS, V), TY = ens_ty(DIMS, V);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ens_stage_kernel<PH, DIMS, TX, TY, 2>, 256,
+                                                   0) != hipSuccess ||
       nb < 1)
     nb = 1;
   return nb * n_cu;
 }
 
-template <typename T>
+template <class PH>
 hipError_t ens_launch(hipStream_t s, int dims, int v, const void* probs, const EnsWork* work,
-                    int nwork, int stage, int grid) {
+                      int nwork, int stage, int grid) {
   if (dims == 1) {
     switch (v) {
-      case 0: return ens_launch_v<T, 1, 0>(s, probs, work, nwork, stage, grid);
-      case 1: return ens_launch_v<T, 1, 1>(s, probs, work, nwork, stage, grid);
-      case 2: return ens_launch_v<T, 1, 2>(s, probs, work, nwork, stage, grid);
+      case 0: return ens_launch_v<PH, 1, 0>(s, probs, work, nwork, stage, grid);
+      case 1: return ens_launch_v<PH, 1, 1>(s, probs, work, nwork, stage, grid);
+      case 2: return ens_launch_v<PH, 1, 2>(s, probs, work, nwork, stage, grid);
     }
     return hipErrorInvalidValue;
   }
   switch (v) {
-    case 0: return ens_launch_v<T, 2, 0>(s, probs, work, nwork, stage, grid);
-    case 1: return ens_launch_v<T, 2, 1>(s, probs, work, nwork, stage, grid);
-    case 2: return ens_launch_v<T, 2, 2>(s, probs, work, nwork, stage, grid);
-    case 3: return ens_launch_v<T, 2, 3>(s, probs, work, nwork, stage, grid);
-    case 4: return ens_launch_v<T, 2, 4>(s, probs, work, nwork, stage, grid);
+    case 0: return ens_launch_v<PH, 2, 0>(s, probs, work, nwork, stage, grid);
+    case 1: return ens_launch_v<PH, 2, 1>(s, probs, work, nwork, stage, grid);
+    case 2: return ens_launch_v<PH, 2, 2>(s, probs, work, nwork, stage, grid);
+    case 3: return ens_launch_v<PH, 2, 3>(s, probs, work, nwork, stage, grid);
+    case 4: return ens_launch_v<PH, 2, 4>(s, probs, work, nwork, stage, grid);
   }
   return hipErrorInvalidValue;
 }
 
-template <typename T>
+template <class PH>
 int ens_grid_of(int dims, int v, int n_cu) {
   if (dims == 1) {
     switch (v) {
-      case 0: return ens_grid_v<T, 1, 0>(n_cu);
-      case 1: return ens_grid_v<T, 1, 1>(n_cu);
-      default: return ens_grid_v<T, 1, 2>(n_cu);
+      case 0: return ens_grid_v<PH, 1, 0>(n_cu);
+      case 1: return ens_grid_v<PH, 1, 1>(n_cu);
+      default: return ens_grid_v<PH, 1, 2>(n_cu);
     }
   }
   switch (v) {
-    case 0: return ens_grid_v<T, 2, 0>(n_cu);
-    case 1: return ens_grid_v<T, 2, 1>(n_cu);
-    case 2: return ens_grid_v<T, 2, 2>(n_cu);
-    case 3: return ens_grid_v<T, 2, 3>(n_cu);
-    default: return ens_grid_v<T, 2, 4>(n_cu);
+    case 0: return ens_grid_v<PH, 2, 0>(n_cu);
+    case 1: return ens_grid_v<PH, 2, 1>(n_cu);
+    case 2: return ens_grid_v<PH, 2, 2>(n_cu);
+    case 3: return ens_grid_v<PH, 2, 3>(n_cu);
+    default: return ens_grid_v<PH, 2, 4>(n_cu);
   }
 }
 

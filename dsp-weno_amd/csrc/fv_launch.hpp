@@ -17,6 +17,7 @@ struct LaunchInfo {
   int fast_rec;   // RV_* id
   int state_fp64; // state held in HBM as double (1) or float (0)
   int mlp_mode;   // MM_* of the mixed-precision DSP-WENO face
+  int scalar_mixed;  // scalar, FP64 state, FP32 SP-WENO jump (ScalarMixedPh)
   GridDev grid;
   hipStream_t stream;
   Ctl* ctl;
@@ -426,6 +427,14 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
                             void* Uo, const StageCoef& k) {
   constexpr int X2 = TECNO_TILE2_X, Y2 = TECNO_TILE2_Y, X1 = TECNO_TILE1_X;
   if (L.system == 0) {
+    if constexpr (sizeof(T) == 8) {
+      if (L.scalar_mixed) {
+        if (L.G != 2) return hipErrorNotSupported;
+        if (L.dims == 1) return launch_stage_t<ScalarMixedPh<2>, 1, X1, 1, 2>(L, P, U, U0, Uo, k);
+        if (L.dims == 2) return launch_stage_t<ScalarMixedPh<2>, 2, X2, Y2, 2>(L, P, U, U0, Uo, k);
+        return hipErrorNotSupported;
+      }
+    }
     if (L.dims == 1)
       return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 1, X1, 1, 3>(L, P, U, U0, Uo, k)
                       : launch_stage_t<ScalarPh<T, 2>, 1, X1, 1, 2>(L, P, U, U0, Uo, k);

@@ -420,6 +420,33 @@ enum { MM_SIMT = 0, MM_BF16 = 1, MM_FP16 = 2 };
 // ln rho, ln beta, Vf (first entropy variable)
 enum { R_RHO = 0, R_UX, R_UY, R_BETA, R_LRHO, R_LBETA, R_VF, R_N };
 
+// Scalar problems on an FP64 state with FP32 reconstruction: state,
+// update and the entropy-conservative part of the lin_/bur_ TeCNO fluxes as
+// ScalarPh<double>, the SP-WENO jump <<u>> of the dissipation term in FP32
+// from the cell jumps taken in FP64 (spw_jump).  Its rounding scales with
+// the jumps, so it stays below the discretisation error on fine meshes,
+// where an FP32 state's own rounding does not.
+template <int G>
+struct ScalarMixedPh : ScalarPh<double, G> {
+  using Base = ScalarPh<double, G>;
+  __device__ static void face(const Phys<double>& P, const void* ldsv, int NB, int base,
+                              int stride, int ax, double F[1], unsigned& nfb) {
+    const double* lds = (const double*)ldsv;
+    double s[2 * G];
+#pragma unroll
+    for (int l = 0; l < 2 * G; ++l) s[l] = lds[base + l * stride];
+    double f = Base::ec_flux(P, s, ax);
+    if (Base::tecno(P)) {
+      const double D = Base::diss_coef(P, s, ax);
+      const float jump = fast::spw_jump(float(s[G - 1] - s[G - 2]), float(s[G] - s[G - 1]),
+                                        float(s[G + 1] - s[G]), float(s[G - 1]), float(s[G]),
+                                        P.corr);
+      f -= 0.5 * D * double(jump);
+    }
+    F[0] = f * P.area[ax];
+  }
+};
+
 template <int RV, int MM>
 struct Euler2Mixed {
   using TS = double;

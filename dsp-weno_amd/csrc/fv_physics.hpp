@@ -21,13 +21,9 @@ struct ScalarPh {
     T u = U[idx];
     ((T*)lds)[c] = (flipx || flipy) ? -u : u;  // scalar wall: ghost = -mirror
   }
-  // face flux: enflo_scalar num_flux.cc:12-57 and lin_/bur_ fluxes
-  __device__ static void face(const Phys<T>& P, const void* ldsv, int NB, int base, int stride,
-                              int ax, T F[1], unsigned& nfb) {
-    const T* lds = (const T*)ldsv;
-    T s[2 * G];
-#pragma unroll
-    for (int l = 0; l < 2 * G; ++l) s[l] = lds[base + l * stride];
+  // entropy-conservative part of the face flux: enflo_scalar num_flux.cc:12-57
+  // and the lin_/bur_ flux files
+  __device__ static T ec_flux(const Phys<T>& P, const T* s, int ax) {
     const T ul = s[G - 1], ur = s[G];
     const T vel = P.vel[ax];
     T f = T(0), f0;
@@ -59,8 +55,24 @@ struct ScalarPh {
         break;
       }
     }
-    if (P.flux == 2 || P.flux == 3 || P.flux == 6 || P.flux == 7) {
-      const T D = (P.flux <= 3) ? m_abs(vel) : T(0.5) * (m_abs(ul) + m_abs(ur));
+    return f;
+  }
+  // TeCNO fluxes: the dissipation coefficient of f - D/2 <<u>>
+  __device__ static bool tecno(const Phys<T>& P) {
+    return P.flux == 2 || P.flux == 3 || P.flux == 6 || P.flux == 7;
+  }
+  __device__ static T diss_coef(const Phys<T>& P, const T* s, int ax) {
+    return (P.flux <= 3) ? m_abs(T(P.vel[ax])) : T(0.5) * (m_abs(s[G - 1]) + m_abs(s[G]));
+  }
+  __device__ static void face(const Phys<T>& P, const void* ldsv, int NB, int base, int stride,
+                              int ax, T F[1], unsigned& nfb) {
+    const T* lds = (const T*)ldsv;
+    T s[2 * G];
+#pragma unroll
+    for (int l = 0; l < 2 * G; ++l) s[l] = lds[base + l * stride];
+    T f = ec_flux(P, s, ax);
+    if (tecno(P)) {
+      const T D = diss_coef(P, s, ax);
       T uij, uji;
       recon_any(P, s, G, uij, uji);
       f -= T(0.5) * D * (uji - uij);

@@ -27,7 +27,7 @@ REC_SECOND, REC_MINMOD, REC_MINABS = 4, 5, 6
 BC = {"periodic": 0, "dirichlet": 1, "neumann": 2, "wall": 3}
 TS_RK1, TS_SSPRK3, TS_JAMESON_RK4, TS_RK4 = 0, 1, 2, 3
 TIME_SCHEME = {"rk1": TS_RK1, "ssprk3": TS_SSPRK3, "jameson_rk4": TS_JAMESON_RK4, "rk4": TS_RK4}
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_STATE64 = 0, 1, 2
 MLP_FP64, MLP_FP32, MLP_BF16, MLP_FP16 = 0, 1, 2, 3
 
 

@@ -85,8 +85,16 @@ enum { TECNO_BC_PERIODIC = 0, TECNO_BC_DIRICHLET = 1, TECNO_BC_NEUMANN = 2,
  * more state-sized device buffer, the accumulated residual.                */
 enum { TECNO_TS_RK1 = 0, TECNO_TS_SSPRK3 = 1, TECNO_TS_JAMESON_RK4 = 2, TECNO_TS_RK4 = 3 };
 
-/* arithmetic of the face kernels */
-enum { TECNO_FP64 = 0, TECNO_FP32 = 1 };
+/* arithmetic of the face kernels.
+ * TECNO_FP32: scalar problems hold and compute their state in FP32; the 2D
+ *   Euler FP32 path keeps state, cell records, the entropy-conservative flux
+ *   and the eigen-projection in FP64 and reconstructs in FP32.
+ * TECNO_FP32_STATE64 (scalar problems with sp_weno / sp_wenoc / corr2): the
+ *   Euler path's split for scalars -- FP64 state, update and
+ *   entropy-conservative flux, the SP-WENO jump in FP32 from the FP64 cell
+ *   jumps.  An FP32 state's rounding (~6e-8 of |u| per step) exceeds
+ *   third-order errors on fine meshes; this keeps the order.               */
+enum { TECNO_FP64 = 0, TECNO_FP32 = 1, TECNO_FP32_STATE64 = 2 };
 
 /* arithmetic of the DSP-WENO network evaluation (sp_weno_dl only) */
 enum { TECNO_MLP_FP64 = 0,   /* SIMT, double                               */
@@ -106,7 +114,7 @@ typedef struct tecno_cfg {
   int32_t spweno_corr;   /* 0 sp_weno, 1 sp_wenoc, 2 sp_weno_corr2           */
   int32_t eno_level;     /* k of "eno k" (2 or 3)                            */
   int32_t time_scheme;   /* TECNO_TS_*                                       */
-  int32_t precision;     /* TECNO_FP64 / TECNO_FP32                          */
+  int32_t precision;     /* TECNO_FP64 / TECNO_FP32 / TECNO_FP32_STATE64     */
   int32_t mlp_dtype;     /* TECNO_MLP_* (sp_weno_dl)                         */
   int32_t dims;          /* active directions: 1, 2 or 3 (leading axes)     */
   int32_t n[3];          /* Nx_cell, Ny_cell, Nz_cell as written in param.in */
@@ -361,7 +369,7 @@ int tecno_ens_advance(tecno_ens* ens, const double* final_times, int t_stamps,
  * are restored afterwards) and keep the fastest. */
 int tecno_ens_autotune(tecno_ens* ens, int steps);
 typedef struct tecno_ens_group {
-  int precision;        /* TECNO_FP64 / TECNO_FP32 */
+  int precision;        /* TECNO_FP64 / TECNO_FP32 / TECNO_FP32_STATE64 */
   int dims;
   int n_members;
   int n_tiles;          /* work items of one stage                          */

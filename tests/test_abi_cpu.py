@@ -51,7 +51,11 @@ def test_create_rejects_bad_configs(bad):
     (dict(recon="minmod", precision=tecno.FP32), "FP32 Euler path"),
     (dict(flux="roe", precision=tecno.FP32), "FP32 Euler path"),
     (dict(model="ns_es", dims=2), "3D grids"),
-    (dict(model="ns_es", dims=3, n=(8, 8, 1), viscosity=("constant", 1e-3)), "at least 2 cells")])
+    (dict(model="ns_es", dims=3, n=(8, 8, 1), viscosity=("constant", 1e-3)), "at least 2 cells"),
+    (dict(precision=tecno.FP32_STATE64), "FP32_STATE64"),
+    (dict(system="scalar", flux="lin_tecno4", recon="eno 2", precision=tecno.FP32_STATE64),
+     "FP32_STATE64"),
+    (dict(precision=3), "precision must be")])
 def test_create_rejects_with_message(kw, msg):
     """The messages name the rule that failed (periodic pairing as
     parameter.cc:570-581, the supported precision combinations, ...)."""

@@ -1,10 +1,11 @@
-"""Batched ensembles of scalar problems: every member ends bitwise (FP64) or
-to FP32 rounding where the same problem run alone ends, with and without the
-captured hipGraph, for every tile shape the autotuner may pick."""
+"""Batched ensembles of scalar problems: every member ends bitwise (FP64 and
+FP32 arithmetic on an FP64 state) or to FP32 rounding (FP32 state) where the
+same problem run alone ends, with and without the captured hipGraph, for
+every tile shape the autotuner may pick."""
 import numpy as np
 import pytest
 
-from tecno import FP32, FP64, Ensemble, Solver, make_cfg
+from tecno import FP32, FP32_STATE64, FP64, Ensemble, Solver, make_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -24,6 +25,12 @@ def problems():
                        precision=FP32), 0.25))
     P.append((make_cfg(system="scalar", flux="bur_tecno4", recon="sp_weno", n=(211, 1, 1), dims=1,
                        lo=(-np.pi, 0, 0), hi=(np.pi, 1, 1), bc="periodic", precision=FP32), 0.5))
+    P.append((make_cfg(system="scalar", flux="lin_tecno4", recon="sp_wenoc", n=(56, 44, 1), dims=2,
+                       lo=(-1, -1, 0), hi=(1, 1, 1), vel=(1, 1, 0), bc="periodic",
+                       precision=FP32_STATE64), 0.25))
+    P.append((make_cfg(system="scalar", flux="bur_tecno4", recon="sp_weno", n=(157, 1, 1), dims=1,
+                       lo=(-np.pi, 0, 0), hi=(np.pi, 1, 1), bc="periodic",
+                       precision=FP32_STATE64), 0.5))
     return P
 
 
@@ -58,7 +65,7 @@ def test_ensemble_members_equal_single_runs(use_graph):
         got = m.get_state()
         tm, itm = m.time()
         assert itm == it_ref and tm == t_ref
-        if c.precision == FP64:
+        if c.precision in (FP64, FP32_STATE64):
             assert np.array_equal(got, st), i
         else:
             assert np.abs(got - st).max() <= 1e-6 * np.abs(st).max(), i
@@ -72,7 +79,7 @@ def test_autotune_picks_a_candidate_and_keeps_results():
         m.set_state(initial(c))
     ens.autotune(steps=2)
     groups = ens.groups()
-    assert len(groups) == 4
+    assert len(groups) == 6
     for g in groups:
         assert g["tile"] in g["tuned_ms"]
         assert all(ms > 0 for ms in g["tuned_ms"].values())
@@ -80,5 +87,5 @@ def test_autotune_picks_a_candidate_and_keeps_results():
     ens.advance([t for _, t in P], t_stamps=2)
     for i, (m, (c, t)) in enumerate(zip(ens.members, P)):
         st, _ = ref[i]
-        if c.precision == FP64:
+        if c.precision in (FP64, FP32_STATE64):
             assert np.array_equal(m.get_state(), st), i

@@ -454,11 +454,13 @@ def tgv_state(n, mach=0.1, gamma=1.4):
     return np.stack([np.ones_like(u), u, v, np.zeros_like(u), p]).reshape(5, -1)
 
 
-def tgv_bench(tecno, n, t_end, recon, net_file, every=0.25, fix_visc=0, cfl=0.5):
+def tgv_bench(tecno, n, t_end, recon, net_file, every=0.25, fix_visc=0, cfl=0.5, fp32=False,
+              mlp=None):
     """Navier-Stokes TGV (ns_es, kepes_tecno4_roe, Re 1600: mu = 1/1600,
-    Pr 0.71, Mach 0.1) in FP64 to t_end: kinetic energy and total entropy
-    (FP64 audit sums per unit volume) every `every` time units, and the
-    throughput of the steps between."""
+    Pr 0.71, Mach 0.1) to t_end, FP64 or (fp32) FP32 reconstruction on the
+    FP64 state with the network arithmetic `mlp`: kinetic energy and total
+    entropy (FP64 audit sums per unit volume) every `every` time units, and
+    the throughput of the steps between."""
     # fix_zmom: the reference's TeCNO fluxes subtract the z-momentum
     # dissipation from the y-momentum (kepes_tecno4_roe.cc:170), which leaves
     # the z momentum undamped; in 3D that grows w to O(1) within t < 0.4 and
@@ -466,7 +468,9 @@ def tgv_bench(tecno, n, t_end, recon, net_file, every=0.25, fix_visc=0, cfl=0.5)
     cfg = tecno.make_cfg(n=(n, n, n), dims=3, lo=(0, 0, 0), hi=(2 * np.pi,) * 3, recon=recon,
                          flux="kepes_tecno4_roe", bc="periodic", cfl=cfl, model="ns_es",
                          viscosity=("constant", 1.0 / 1600.0), prandtl=0.71, fix_zmom=1,
-                         fix_visc=fix_visc)
+                         fix_visc=fix_visc, precision=tecno.FP32 if fp32 else tecno.FP64,
+                         mlp_dtype=mlp if mlp is not None else tecno.MLP_FP64,
+                         sign_fallback=1 if fp32 else 0)
     s = tecno.Solver(cfg)
     if recon == "sp_weno_dl":
         s.load_net(net_file)
@@ -536,6 +540,10 @@ def main():
     ap.add_argument("--fix-visc", action="store_true",
                     help="--tgv: keep the K21/K31/K32 viscous blocks the reference overwrites")
     ap.add_argument("--cfl", type=float, default=0.5, help="--tgv: CFL number")
+    ap.add_argument("--fp32", action="store_true",
+                    help="--tgv: FP32 reconstruction on the FP64 state")
+    ap.add_argument("--mlp", default="bf16", choices=["fp32", "bf16", "fp16"],
+                    help="--tgv --fp32 sp_weno_dl: network arithmetic")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one plain 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -558,12 +566,17 @@ def main():
 
     if args.tgv:
         n = args.n if args.n != 2048 else 256
+        mlp = {"fp32": tecno.MLP_FP32, "bf16": tecno.MLP_BF16, "fp16": tecno.MLP_FP16}[args.mlp]
         r = tgv_bench(tecno, n, args.t_end, args.tgv, args.net, fix_visc=int(args.fix_visc),
-                      cfl=args.cfl)
+                      cfl=args.cfl, fp32=args.fp32, mlp=mlp if args.fp32 else None)
+        arith = "fp64"
+        if args.fp32:
+            arith = ("fp64 state, flux, eigensystem and viscous terms; fp32 reconstruction" +
+                     (f", {args.mlp} network" if args.tgv == "sp_weno_dl" else ""))
         print(json.dumps({"metric": "cell-updates/s, 3D Navier-Stokes Taylor-Green vortex",
                           "value": r["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1,
                           "higher_is_better": True, "data": "synthetic (analytic initial state)",
-                          "dtype": "fp64", "net_file": os.path.basename(args.net)
+                          "dtype": arith, "net_file": os.path.basename(args.net)
                           if args.tgv == "sp_weno_dl" else None,
                           "config": {"workload": f"TGV {n}^3 periodic [0,2pi]^3, Re 1600, Mach 0.1, "
                                                  f"ns_es, kepes_tecno4_roe (z-momentum dissipation "

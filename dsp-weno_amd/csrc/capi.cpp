@@ -184,7 +184,17 @@ static void refresh_phys(tecno_ctx* c) {
   c->P32.net.b = c->db32;
   c->P32.mfma_w[0] = c->dWmf;
   c->P32.mfma_w[1] = c->dWmf ? c->dWmf + 4 * 64 * 8 : nullptr;
-  c->P64.mfma_w[0] = c->P64.mfma_w[1] = nullptr;
+  // the 3D FP32 face runs on the FP64 state (Phys<double>) with the FP32
+  // and matrix-core weights
+  c->P64.mfma_w[0] = c->P32.mfma_w[0];
+  c->P64.mfma_w[1] = c->P32.mfma_w[1];
+  for (Phys<double>* P : {&c->P64}) {
+    P->net32.nl = c->P32.net.nl;
+    for (int i = 0; i <= TECNO_NET_MAXL; ++i) P->net32.w[i] = c->P32.net.w[i];
+    P->net32.W = c->dW32;
+    P->net32.b = c->db32;
+  }
+  c->P32.net32 = c->P32.net;
 }
 
 static int ensure_scratch(tecno_ctx* c, size_t bytes) {
@@ -248,7 +258,8 @@ static int upload_net(tecno_ctx* c) {
 // the FP64 kernel family: FP64, and scalar FP32 arithmetic on an FP64 state
 // (whose stage kernel differs in the face only, L.scalar_mixed)
 static bool fp64(const tecno_ctx* c) {
-  return c->cfg.precision == TECNO_FP64 || c->cfg.precision == TECNO_FP32_STATE64;
+  return c->cfg.precision == TECNO_FP64 || c->cfg.precision == TECNO_FP32_STATE64 ||
+         (c->cfg.system == TECNO_SYS_EULER && c->cfg.dims == 3);
 }
 
 // cells the reference stores per held cell: the param.in cell counts of the
@@ -312,8 +323,16 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   if (g.system == TECNO_SYS_EULER && g.dims == 1)
     return fail(TECNO_EINVAL, "Euler: 2D or 3D grids");
   if (g.system == TECNO_SYS_EULER && g.dims == 3) {
-    if (g.precision != TECNO_FP64)
-      return fail(TECNO_EINVAL, "3D Euler / Navier-Stokes runs in FP64 only");
+    // the 3D FP32 path reconstructs in FP32 on the FP64 state: TeCNO fluxes
+    // of the kepec / roe_ec / kep family with sp_weno (any correction) or
+    // sp_weno_dl
+    if (g.precision == TECNO_FP32 &&
+        (g.flux < 100 || g.flux > 106 ||
+         (g.recon != TECNO_REC_SP_WENO && g.recon != TECNO_REC_SP_WENO_DL)))
+      return fail(TECNO_EINVAL, "3D FP32 path: TeCNO fluxes (kepes_tecno*, roe_tecno*, "
+                                "kep_tecno_roe) with sp_weno or sp_weno_dl only");
+    if (g.precision != TECNO_FP64 && g.precision != TECNO_FP32)
+      return fail(TECNO_EINVAL, "3D Euler / Navier-Stokes: precision FP64 or FP32");
     if (g.recon == TECNO_REC_ENO && g.eno_level == 3)
       return fail(TECNO_EINVAL, "3D: reconstructions with 2 ghost cells (eno 3 is 2D only)");
   }
@@ -421,6 +440,12 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   c->L.G = c->G;
   c->L.fast_flux = g.flux == TECNO_FLUX_KEPES_TECNO4_ROE && !g.ec1_fix ? FV_KEPES4_ROE : FV_GENERIC;
   c->L.scalar_mixed = g.precision == TECNO_FP32_STATE64 ? 1 : 0;
+  c->L.mix3 = M3_NONE;
+  if (g.system == TECNO_SYS_EULER && g.dims == 3 && g.precision == TECNO_FP32)
+    c->L.mix3 = g.recon == TECNO_REC_SP_WENO        ? M3_SPW
+                : g.mlp_dtype == TECNO_MLP_BF16 ? M3_DSP_BF16
+                : g.mlp_dtype == TECNO_MLP_FP16 ? M3_DSP_FP16
+                                                : M3_DSP_SIMT;
   c->L.mlp_mode = g.mlp_dtype == TECNO_MLP_BF16 ? MM_BF16
                   : g.mlp_dtype == TECNO_MLP_FP16 ? MM_FP16
                                                    : MM_SIMT;
@@ -1333,6 +1358,7 @@ struct EnsGroupH {
   EnsWork* dwork = nullptr;
   int nwork = 0;
   int grid = 0;
+  int gridf = 1;  // workgroups launched per resident one (the stride loop takes any count)
   double tuned_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 }  // namespace
@@ -1376,7 +1402,7 @@ static int ens_build_work(tecno_ens* e, EnsGroupH& G) {
   HIPCHK(hipMalloc(&G.dwork, w.size() * sizeof(EnsWork)));
   HIPCHK(hipMemcpy(G.dwork, w.data(), w.size() * sizeof(EnsWork), hipMemcpyHostToDevice));
   const int resident = ens_grid(G.prec, G.dims, G.variant, e->n_cu);
-  G.grid = std::max(1, std::min(G.nwork, resident));
+  G.grid = std::max(1, std::min(G.nwork, resident * G.gridf));
   ens_drop_graph(e);
   return 0;
 }
@@ -1633,31 +1659,40 @@ int tecno_ens_autotune(tecno_ens* e, int steps) {
   for (int gi = 0; gi < (int)e->g.size() && !rc; ++gi) {
     EnsGroupH& G = e->g[gi];
     double best = 1e300;
-    int bestv = 0;
+    int bestv = 0, bestf = 1;
+    // tile shape x workgroups per resident slot (1: one persistent wave of
+    // workgroups as the occupancy query counts them; 2, 4: more, which the
+    // hardware schedules as slots free up)
     for (int v = 0; v < ens_nvariants(G.dims) && !rc; ++v) {
-      G.variant = v;
-      if ((rc = ens_build_work(e, G)) || (rc = restore())) break;
-      if (ens_step(e, gi) != hipSuccess) {  // warm-up step, not timed
-        rc = fail(TECNO_EHIP, "autotune launch failed");
-        break;
-      }
-      (void)hipEventRecord(e0, e->s);
-      for (int k = 0; k < steps; ++k) (void)ens_step(e, gi);
-      (void)hipEventRecord(e1, e->s);
-      if (hipEventSynchronize(e1) != hipSuccess) {
-        rc = fail(TECNO_EHIP, "autotune: kernel failed");
-        break;
-      }
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      G.tuned_ms[v] = ms / steps;
-      if (ms < best) {
-        best = ms;
-        bestv = v;
+      G.tuned_ms[v] = 0.0;
+      for (int f = 1; f <= 4 && !rc; f *= 2) {
+        G.variant = v;
+        G.gridf = f;
+        if ((rc = ens_build_work(e, G)) || (rc = restore())) break;
+        if (ens_step(e, gi) != hipSuccess) {  // warm-up step, not timed
+          rc = fail(TECNO_EHIP, "autotune launch failed");
+          break;
+        }
+        (void)hipEventRecord(e0, e->s);
+        for (int k = 0; k < steps; ++k) (void)ens_step(e, gi);
+        (void)hipEventRecord(e1, e->s);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+          rc = fail(TECNO_EHIP, "autotune: kernel failed");
+          break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (G.tuned_ms[v] == 0.0 || ms / steps < G.tuned_ms[v]) G.tuned_ms[v] = ms / steps;
+        if (ms < best) {
+          best = ms;
+          bestv = v;
+          bestf = f;
+        }
       }
     }
     if (!rc) {
       G.variant = bestv;
+      G.gridf = bestf;
       rc = ens_build_work(e, G);
     }
   }

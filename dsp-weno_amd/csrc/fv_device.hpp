@@ -418,6 +418,7 @@ struct Phys {
   int mu_model;       // 0 constant, 1 sutherland, 2 power
   double mu_ref, T_ref, T_0, omega, prandtl, Cp;
   NetView<T> net;
+  NetView<float> net32;  // FP32 weights, for the FP32 network of the FP64-state 3D face
 };
 
 // reconstruct one component on a stencil of 2*ng values.  RS selects the

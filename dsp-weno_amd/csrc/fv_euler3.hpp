@@ -11,14 +11,130 @@
 // tile, one per lane; both go to LDS over the dead records; each cell then
 // gathers its 6 faces and 8 corner nodes, applies the SSPRK3 update and
 // reduces the CFL speed.
+//
+// With MIX != M3_NONE (the FP32 path, Euler3Ph<double, 2, 0, MIX>) the face
+// reconstructs the five characteristic jumps in FP32 jump form, as the 2D
+// mixed face does (fv_mixed.hpp); state, records, entropy-conservative flux,
+// eigensystem, dissipation product and viscous terms stay FP64.
 #pragma once
 #include "fv_kernels.hpp"
+#include "fv_mixed.hpp"
 
 namespace tecno {
 
 // record layout, double [E3_NREC][NB]: primitive, beta as kepec_flux computes
 // it, entropy variables (fv.cc:421-431)
 enum { E3_RHO = 0, E3_U, E3_V, E3_W, E3_P, E3_BK, E3_VF, E3_V1, E3_V2, E3_V3, E3_VL, E3_NREC };
+
+// reconstruction arithmetic of the 3D face
+enum { M3_NONE = 0, M3_SPW = 1, M3_DSP_SIMT = 2, M3_DSP_BF16 = 3, M3_DSP_FP16 = 4 };
+
+__device__ __forceinline__ float sel5(int i, float a, float b, float c, float d, float e) {
+  return i == 0 ? a : i == 1 ? b : i == 2 ? c : i == 3 ? d : e;
+}
+
+// Reconstructed jumps <<Z_i>> of the five components of one face in FP32
+// jump form.  The differences of the FP64 entropy variables of neighbouring
+// cells are projected on the scaled eigenvectors Rm in FP64 (a projected
+// jump can be a near-total cancellation of its terms), then SP-WENO's jump
+// (limiter.h:193-252) or DSP-WENO's (neural_network.cc:16-77, 320-325) is
+// formed in FP32 from the three cell jumps.  The network runs on the vector
+// units (M3_DSP_SIMT) or as one v_mfma_f32_32x32x16 per layer for the 64
+// lanes of the wave (all lanes call this together).  With sign_fallback a
+// network jump whose sign disagrees with the cell jump is replaced by
+// SP-WENO's; nfb counts the replacements.
+template <int MIX>
+__device__ __forceinline__ void jumps3_fp32(const Phys<double>& P, const double* ld, int NB,
+                                            int base, int stride, const double Rm[5][5],
+                                            double dZ[5], unsigned& nfb) {
+  float d0[5], d1[5], d2[5], z1[5];
+  {
+    double V[4][5];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int cc = base + l * stride;
+#pragma unroll
+      for (int r = 0; r < 5; ++r) V[l][r] = ld[(E3_VF + r) * NB + cc];
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, zz = 0.0;
+#pragma unroll
+      for (int r = 0; r < 5; ++r) {
+        a0 += Rm[r][i] * (V[1][r] - V[0][r]);
+        a1 += Rm[r][i] * (V[2][r] - V[1][r]);
+        a2 += Rm[r][i] * (V[3][r] - V[2][r]);
+        zz += Rm[r][i] * V[1][r];
+      }
+      d0[i] = (float)a0;
+      d1[i] = (float)a1;
+      d2[i] = (float)a2;
+      z1[i] = (float)zz;
+    }
+  }
+  float jmp[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  unsigned bad = 0;
+  if constexpr (MIX == M3_SPW) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) jmp[i] = fast::spw_jump(d0[i], d1[i], d2[i], z1[i], z1[i] + d1[i], P.corr);
+  } else {
+#ifndef TECNO_HOST_BUILD
+    [[maybe_unused]] mfma::Net5<MIX == M3_DSP_FP16> net;
+    if constexpr (MIX != M3_DSP_SIMT) net.load(P.mfma_w[MIX == M3_DSP_FP16 ? 1 : 0], threadIdx.x & 63);
+#endif
+#pragma unroll 1
+    for (int i = 0; i < 5; ++i) {
+      const float e1 = sel5(i, d1[0], d1[1], d1[2], d1[3], d1[4]);
+      const bool skip = !(fabsf(e1) >= 1e-15f);
+#ifndef TECNO_HOST_BUILD
+      if (__all(skip)) continue;  // zero centre jump on every lane: jump 0
+#endif
+      const float e0 = sel5(i, d0[0], d0[1], d0[2], d0[3], d0[4]);
+      const float e2 = sel5(i, d2[0], d2[1], d2[2], d2[3], d2[4]);
+      const float y1 = sel5(i, z1[0], z1[1], z1[2], z1[3], z1[4]);
+      const float y2 = y1 + e1, y0 = y1 - e0, y3 = y2 + e2;
+      const float maxabs = fmaxf(fmaxf(fabsf(y0), fabsf(y1)), fmaxf(fabsf(y2), fabsf(y3)));
+      float st[5], tm, tp, maxd;
+      fast::features_nv(e0, e1, e2, maxabs, st, tm, tp, maxd);
+      float j;
+#ifndef TECNO_HOST_BUILD
+      if constexpr (MIX != M3_DSP_SIMT) {
+        // a skipped lane feeds zeros: its partner lane in the matrix column
+        // must not see non-finite ratios
+#pragma unroll
+        for (int k = 0; k < 5; ++k) st[k] = skip ? 0.0f : st[k];
+        float lg[5];
+        net.eval(st, lg);
+        j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
+      } else
+#endif
+      {
+        fast::body5(P.net32, st);
+        j = fast::sign_round(fast::head_combine(st, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
+      }
+      if (skip) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
+      bad |= (j * e1 < 0.0f) ? (1u << i) : 0u;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) jmp[k] = (k == i) ? j : jmp[k];
+    }
+    if (!P.sign_fallback) bad = 0;
+    while (bad) {
+      const int i = __builtin_ctz(bad);
+      bad &= bad - 1;
+      const float e0 = sel5(i, d0[0], d0[1], d0[2], d0[3], d0[4]);
+      const float e1 = sel5(i, d1[0], d1[1], d1[2], d1[3], d1[4]);
+      const float e2 = sel5(i, d2[0], d2[1], d2[2], d2[3], d2[4]);
+      const float y1 = sel5(i, z1[0], z1[1], z1[2], z1[3], z1[4]);
+      const float j = fast::spw_jump(e0, e1, e2, y1, y1 + e1, 0);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) jmp[k] = (k == i) ? j : jmp[k];
+      ++nfb;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) dZ[i] = (double)jmp[i];
+}
+
 
 template <typename T>
 struct Pr3 {
@@ -193,8 +309,10 @@ __device__ __forceinline__ void roe3(const Phys<T>& P, const Pr3<T>& l, const Pr
 // FAM 2: as FAM 0 with second, minmod or minabs.
 // Separate instantiations keep the face of the common case inlined and its
 // register pressure unchanged by the rarely used options.
-template <typename T, int G, int FAM = 0>
+template <typename T, int G, int FAM = 0, int MIX = M3_NONE>
 struct Euler3Ph {
+  static_assert(MIX == M3_NONE || (sizeof(T) == 8 && G == 2 && FAM == 0),
+                "the FP32 reconstruction runs on the FP64 state with the TeCNO family");
   using TS = T;
   using TF = T;
   using TP = T;
@@ -244,6 +362,11 @@ struct Euler3Ph {
   // FV::num_flux for one face from the 2G cells base + l*stride
   __device__ static void face(const Phys<T>& P, const T* ld, int NB, int base, int stride, int ax,
                               T F[5]) {
+    unsigned nfb = 0;
+    face(P, ld, NB, base, stride, ax, F, nfb);
+  }
+  __device__ static void face(const Phys<T>& P, const T* ld, int NB, int base, int stride, int ax,
+                              T F[5], unsigned& nfb) {
     constexpr int S = 2 * G;
     const int L = G - 1, R = G;
     T n[3] = {T(0), T(0), T(0)};
@@ -345,6 +468,9 @@ struct Euler3Ph {
     }
     // Z = R~^T V on the stencil, reconstruct, Diff = R~ Lambda dZ
     T dZ[5];
+    if constexpr (MIX != M3_NONE) {
+      jumps3_fp32<MIX>(P, ld, NB, base, stride, Rm, dZ, nfb);
+    } else {
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
       T st[S];
@@ -358,6 +484,7 @@ struct Euler3Ph {
       T zl, zr;
       recon_any<FAM == 2 ? 2 : 1>(P, st, G, zl, zr);
       dZ[i] = zr - zl;
+    }
     }
     T Diff[5];
 #pragma unroll
@@ -562,7 +689,8 @@ struct Tile3 {
     }
   }
 
-  __device__ static void faces(const Phys<T>& P, const T* ld, int tid, T Fr[ROUNDS][NV]) {
+  __device__ static void faces(const Phys<T>& P, const T* ld, int tid, T Fr[ROUNDS][NV],
+                               unsigned& nfb) {
 #pragma unroll
     for (int rd = 0; rd < ROUNDS; ++rd) {
       int f = rd * 256 + tid;
@@ -587,7 +715,11 @@ struct Tile3 {
         stride = BX * BY;
         ax = 2;
       }
-      PH::face(P, ld, NB, base, stride, ax, Fr[rd]);
+      // lanes past the end evaluate the last face again (the whole wave
+      // takes part in the matrix-core network); their count is not kept
+      unsigned nf = 0;
+      PH::face(P, ld, NB, base, stride, ax, Fr[rd], nf);
+      if (rd * 256 + tid < NF) nfb += nf;
     }
   }
 
@@ -688,7 +820,9 @@ void stage3_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDe
   TL::load(A, P, g, ld, tid, x0, y0, z0);
   __syncthreads();
   T Fr[TL::ROUNDS][TL::NV];
-  TL::faces(P, ld, tid, Fr);
+  unsigned nfb = 0;
+  TL::faces(P, ld, tid, Fr, nfb);
+  if (nfb) atomicAdd(&A.ctl->nfallback, (unsigned long long)nfb);
   T X[4], Y[4], Z[4];
   if (VISC && tid < TL::NN) TL::node(P, g, ld, tid, x0, y0, z0, X, Y, Z);
   __syncthreads();

@@ -18,6 +18,7 @@ struct LaunchInfo {
   int state_fp64; // state held in HBM as double (1) or float (0)
   int mlp_mode;   // MM_* of the mixed-precision DSP-WENO face
   int scalar_mixed;  // scalar, FP64 state, FP32 SP-WENO jump (ScalarMixedPh)
+  int mix3;          // M3_* reconstruction of the 3D face (M3_NONE: FP64)
   GridDev grid;
   hipStream_t stream;
   Ctl* ctl;
@@ -448,6 +449,22 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
     if constexpr (sizeof(T) == 8) {
       const bool visc = P.model == 11 || P.model == 12;
       if (L.G != 2) return hipErrorNotSupported;
+      switch (L.mix3) {
+        case M3_SPW:
+          if (visc) return launch_stage3_t<Euler3Ph<T, 2, 0, M3_SPW>, 2, true>(L, P, U, U0, Uo, k);
+          return launch_stage3_t<Euler3Ph<T, 2, 0, M3_SPW>, 2, false>(L, P, U, U0, Uo, k);
+        case M3_DSP_SIMT:
+          if (visc) return launch_stage3_t<Euler3Ph<T, 2, 0, M3_DSP_SIMT>, 2, true>(L, P, U, U0, Uo, k);
+          return launch_stage3_t<Euler3Ph<T, 2, 0, M3_DSP_SIMT>, 2, false>(L, P, U, U0, Uo, k);
+        case M3_DSP_BF16:
+          if (visc) return launch_stage3_t<Euler3Ph<T, 2, 0, M3_DSP_BF16>, 2, true>(L, P, U, U0, Uo, k);
+          return launch_stage3_t<Euler3Ph<T, 2, 0, M3_DSP_BF16>, 2, false>(L, P, U, U0, Uo, k);
+        case M3_DSP_FP16:
+          if (visc) return launch_stage3_t<Euler3Ph<T, 2, 0, M3_DSP_FP16>, 2, true>(L, P, U, U0, Uo, k);
+          return launch_stage3_t<Euler3Ph<T, 2, 0, M3_DSP_FP16>, 2, false>(L, P, U, U0, Uo, k);
+        default:
+          break;
+      }
       if (P.flux >= 125) {
         if (visc) return launch_stage3_t<Euler3Ph<T, 2, 1>, 2, true>(L, P, U, U0, Uo, k);
         return launch_stage3_t<Euler3Ph<T, 2, 1>, 2, false>(L, P, U, U0, Uo, k);

@@ -499,7 +499,8 @@ class Ensemble:
         for g in range(lib().tecno_ens_num_groups(self.h)):
             info = EnsGroup()
             _check(lib().tecno_ens_group_info(self.h, g, C.byref(info)))
-            out.append(dict(precision="fp32" if info.precision == FP32 else "fp64", dims=info.dims,
+            label = {FP64: "fp64", FP32: "fp32", FP32_STATE64: "fp32_state64"}[info.precision]
+            out.append(dict(precision=label, dims=info.dims,
                             members=info.n_members, tiles=info.n_tiles, grid=info.grid,
                             tile=(info.tile_x, info.tile_y),
                             tuned_ms={(info.cand_x[v], info.cand_y[v]): info.tuned_ms[v]

@@ -373,11 +373,13 @@ typedef struct tecno_ens_group {
   int dims;
   int n_members;
   int n_tiles;          /* work items of one stage                          */
-  int grid;             /* persistent workgroups                            */
+  int grid;             /* workgroups launched (striding over the tiles)    */
   int tile_x, tile_y;   /* chosen tile shape                                */
   int n_candidates;
   int cand_x[8], cand_y[8];
-  double tuned_ms[8];   /* autotuner time per candidate (0 = not timed)     */
+  double tuned_ms[8];   /* autotuner time per candidate tile shape, best   *
+                         * over 1, 2, 4 workgroups per resident slot       *
+                         * (0 = not timed)                                 */
 } tecno_ens_group;
 int tecno_ens_num_groups(const tecno_ens* ens);
 int tecno_ens_group_info(const tecno_ens* ens, int g, tecno_ens_group* out);

@@ -55,7 +55,9 @@ def test_create_rejects_bad_configs(bad):
     (dict(precision=tecno.FP32_STATE64), "FP32_STATE64"),
     (dict(system="scalar", flux="lin_tecno4", recon="eno 2", precision=tecno.FP32_STATE64),
      "FP32_STATE64"),
-    (dict(precision=3), "precision must be")])
+    (dict(precision=3), "precision must be"),
+    (dict(dims=3, n=(8, 8, 8), flux="roe", precision=tecno.FP32), "3D FP32 path"),
+    (dict(dims=3, n=(8, 8, 8), recon="eno 2", precision=tecno.FP32), "3D FP32 path")])
 def test_create_rejects_with_message(kw, msg):
     """The messages name the rule that failed (periodic pairing as
     parameter.cc:570-581, the supported precision combinations, ...)."""

@@ -255,11 +255,19 @@ static int upload_net(tecno_ctx* c) {
   return 0;
 }
 
-// the FP64 kernel family: FP64, and scalar FP32 arithmetic on an FP64 state
-// (whose stage kernel differs in the face only, L.scalar_mixed)
+// 2D Euler FP32 with a TeCNO flux other than the fused kepes_tecno4_roe
+// stage (or with ec1_fix): the generic face with FP32 jumps, FP64 kernel family
+static bool euler2_generic_fp32(const tecno_cfg& g) {
+  return g.system == TECNO_SYS_EULER && g.dims == 2 && g.precision == TECNO_FP32 &&
+         (g.flux != TECNO_FLUX_KEPES_TECNO4_ROE || g.ec1_fix);
+}
+
+// the FP64 kernel family: FP64, and the FP32 paths that keep an FP64 state
+// and differ from FP64 in the face only (scalar L.scalar_mixed, 3D L.mix3,
+// 2D generic TeCNO L.mix2)
 static bool fp64(const tecno_ctx* c) {
   return c->cfg.precision == TECNO_FP64 || c->cfg.precision == TECNO_FP32_STATE64 ||
-         (c->cfg.system == TECNO_SYS_EULER && c->cfg.dims == 3);
+         (c->cfg.system == TECNO_SYS_EULER && c->cfg.dims == 3) || euler2_generic_fp32(c->cfg);
 }
 
 // cells the reference stores per held cell: the param.in cell counts of the
@@ -402,13 +410,13 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   // the mixed-precision Euler path keeps the state in FP64
   c->L.state_fp64 = (fp64(c) || g.system == TECNO_SYS_EULER) ? 1 : 0;
   c->esize = c->L.state_fp64 ? 8 : 4;
-  if (!fp64(c) && g.system == TECNO_SYS_EULER &&
-      (g.flux != TECNO_FLUX_KEPES_TECNO4_ROE || g.ec1_fix ||
-       (g.recon != TECNO_REC_SP_WENO && g.recon != TECNO_REC_SP_WENO_DL) ||
-       g.spweno_corr == 2 && g.recon != TECNO_REC_SP_WENO)) {
+  if (g.system == TECNO_SYS_EULER && g.dims == 2 && g.precision == TECNO_FP32 &&
+      (g.flux < 100 || g.flux > 106 ||
+       (g.recon != TECNO_REC_SP_WENO && g.recon != TECNO_REC_SP_WENO_DL))) {
     delete c;
-    return fail(TECNO_EINVAL, "FP32 Euler path: kepes_tecno4_roe with sp_weno, sp_wenoc or "
-                              "sp_weno_dl only (use FP64 for the other options)");
+    return fail(TECNO_EINVAL, "FP32 Euler path: TeCNO fluxes (kepes_tecno*, roe_tecno*, "
+                              "kep_tecno_roe) with sp_weno, sp_wenoc, corr2 or sp_weno_dl only "
+                              "(use FP64 for the other options)");
   }
   if (hipGetDevice(&c->device) != hipSuccess) {
     delete c;
@@ -440,12 +448,15 @@ int tecno_create(const tecno_cfg* cfg, tecno_ctx** out) {
   c->L.G = c->G;
   c->L.fast_flux = g.flux == TECNO_FLUX_KEPES_TECNO4_ROE && !g.ec1_fix ? FV_KEPES4_ROE : FV_GENERIC;
   c->L.scalar_mixed = g.precision == TECNO_FP32_STATE64 ? 1 : 0;
-  c->L.mix3 = M3_NONE;
-  if (g.system == TECNO_SYS_EULER && g.dims == 3 && g.precision == TECNO_FP32)
-    c->L.mix3 = g.recon == TECNO_REC_SP_WENO        ? M3_SPW
-                : g.mlp_dtype == TECNO_MLP_BF16 ? M3_DSP_BF16
-                : g.mlp_dtype == TECNO_MLP_FP16 ? M3_DSP_FP16
-                                                : M3_DSP_SIMT;
+  c->L.mix3 = c->L.mix2 = M3_NONE;
+  {
+    const int mix = g.recon == TECNO_REC_SP_WENO        ? M3_SPW
+                    : g.mlp_dtype == TECNO_MLP_BF16 ? M3_DSP_BF16
+                    : g.mlp_dtype == TECNO_MLP_FP16 ? M3_DSP_FP16
+                                                    : M3_DSP_SIMT;
+    if (g.system == TECNO_SYS_EULER && g.dims == 3 && g.precision == TECNO_FP32) c->L.mix3 = mix;
+    if (euler2_generic_fp32(g)) c->L.mix2 = mix;
+  }
   c->L.mlp_mode = g.mlp_dtype == TECNO_MLP_BF16 ? MM_BF16
                   : g.mlp_dtype == TECNO_MLP_FP16 ? MM_FP16
                                                    : MM_SIMT;

@@ -26,23 +26,11 @@ namespace tecno {
 // it, entropy variables (fv.cc:421-431)
 enum { E3_RHO = 0, E3_U, E3_V, E3_W, E3_P, E3_BK, E3_VF, E3_V1, E3_V2, E3_V3, E3_VL, E3_NREC };
 
-// reconstruction arithmetic of the 3D face
-enum { M3_NONE = 0, M3_SPW = 1, M3_DSP_SIMT = 2, M3_DSP_BF16 = 3, M3_DSP_FP16 = 4 };
-
-__device__ __forceinline__ float sel5(int i, float a, float b, float c, float d, float e) {
-  return i == 0 ? a : i == 1 ? b : i == 2 ? c : i == 3 ? d : e;
-}
-
-// Reconstructed jumps <<Z_i>> of the five components of one face in FP32
-// jump form.  The differences of the FP64 entropy variables of neighbouring
-// cells are projected on the scaled eigenvectors Rm in FP64 (a projected
-// jump can be a near-total cancellation of its terms), then SP-WENO's jump
-// (limiter.h:193-252) or DSP-WENO's (neural_network.cc:16-77, 320-325) is
-// formed in FP32 from the three cell jumps.  The network runs on the vector
-// units (M3_DSP_SIMT) or as one v_mfma_f32_32x32x16 per layer for the 64
-// lanes of the wave (all lanes call this together).  With sign_fallback a
-// network jump whose sign disagrees with the cell jump is replaced by
-// SP-WENO's; nfb counts the replacements.
+// 3D face: the five characteristic jumps in FP32 jump form.  The
+// differences of the FP64 entropy variables of neighbouring cells are
+// projected on the scaled eigenvectors Rm in FP64 (a projected jump can be a
+// near-total cancellation of its terms); jumps_fp32 (fv_mixed.hpp) forms
+// SP-WENO's or DSP-WENO's jump in FP32.
 template <int MIX>
 __device__ __forceinline__ void jumps3_fp32(const Phys<double>& P, const double* ld, int NB,
                                             int base, int stride, const double Rm[5][5],
@@ -72,65 +60,8 @@ __device__ __forceinline__ void jumps3_fp32(const Phys<double>& P, const double*
       z1[i] = (float)zz;
     }
   }
-  float jmp[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  unsigned bad = 0;
-  if constexpr (MIX == M3_SPW) {
-#pragma unroll
-    for (int i = 0; i < 5; ++i) jmp[i] = fast::spw_jump(d0[i], d1[i], d2[i], z1[i], z1[i] + d1[i], P.corr);
-  } else {
-#ifndef TECNO_HOST_BUILD
-    [[maybe_unused]] mfma::Net5<MIX == M3_DSP_FP16> net;
-    if constexpr (MIX != M3_DSP_SIMT) net.load(P.mfma_w[MIX == M3_DSP_FP16 ? 1 : 0], threadIdx.x & 63);
-#endif
-#pragma unroll 1
-    for (int i = 0; i < 5; ++i) {
-      const float e1 = sel5(i, d1[0], d1[1], d1[2], d1[3], d1[4]);
-      const bool skip = !(fabsf(e1) >= 1e-15f);
-#ifndef TECNO_HOST_BUILD
-      if (__all(skip)) continue;  // zero centre jump on every lane: jump 0
-#endif
-      const float e0 = sel5(i, d0[0], d0[1], d0[2], d0[3], d0[4]);
-      const float e2 = sel5(i, d2[0], d2[1], d2[2], d2[3], d2[4]);
-      const float y1 = sel5(i, z1[0], z1[1], z1[2], z1[3], z1[4]);
-      const float y2 = y1 + e1, y0 = y1 - e0, y3 = y2 + e2;
-      const float maxabs = fmaxf(fmaxf(fabsf(y0), fabsf(y1)), fmaxf(fabsf(y2), fabsf(y3)));
-      float st[5], tm, tp, maxd;
-      fast::features_nv(e0, e1, e2, maxabs, st, tm, tp, maxd);
-      float j;
-#ifndef TECNO_HOST_BUILD
-      if constexpr (MIX != M3_DSP_SIMT) {
-        // a skipped lane feeds zeros: its partner lane in the matrix column
-        // must not see non-finite ratios
-#pragma unroll
-        for (int k = 0; k < 5; ++k) st[k] = skip ? 0.0f : st[k];
-        float lg[5];
-        net.eval(st, lg);
-        j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
-      } else
-#endif
-      {
-        fast::body5(P.net32, st);
-        j = fast::sign_round(fast::head_combine(st, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
-      }
-      if (skip) j = 0.0f;  // reconstruct_sp_weno_dl zero-jump rule
-      bad |= (j * e1 < 0.0f) ? (1u << i) : 0u;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) jmp[k] = (k == i) ? j : jmp[k];
-    }
-    if (!P.sign_fallback) bad = 0;
-    while (bad) {
-      const int i = __builtin_ctz(bad);
-      bad &= bad - 1;
-      const float e0 = sel5(i, d0[0], d0[1], d0[2], d0[3], d0[4]);
-      const float e1 = sel5(i, d1[0], d1[1], d1[2], d1[3], d1[4]);
-      const float e2 = sel5(i, d2[0], d2[1], d2[2], d2[3], d2[4]);
-      const float y1 = sel5(i, z1[0], z1[1], z1[2], z1[3], z1[4]);
-      const float j = fast::spw_jump(e0, e1, e2, y1, y1 + e1, 0);
-#pragma unroll
-      for (int k = 0; k < 5; ++k) jmp[k] = (k == i) ? j : jmp[k];
-      ++nfb;
-    }
-  }
+  float jmp[5];
+  jumps_fp32<MIX, 5>(P, d0, d1, d2, z1, jmp, nfb);
 #pragma unroll
   for (int i = 0; i < 5; ++i) dZ[i] = (double)jmp[i];
 }

@@ -19,6 +19,7 @@ struct LaunchInfo {
   int mlp_mode;   // MM_* of the mixed-precision DSP-WENO face
   int scalar_mixed;  // scalar, FP64 state, FP32 SP-WENO jump (ScalarMixedPh)
   int mix3;          // M3_* reconstruction of the 3D face (M3_NONE: FP64)
+  int mix2;          // M3_* of the 2D generic TeCNO face on the FP64 state (0: FP64)
   GridDev grid;
   hipStream_t stream;
   Ctl* ctl;
@@ -492,6 +493,22 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
       return launch_stage_t<Euler2Mixed<RV_DSP, MM_FP16>, 2, XM, YM, 2>(L, P, U, U0, Uo, k);
     return launch_stage_t<Euler2Mixed<RV_DSP, MM_SIMT>, 2, XM, YM, 2>(L, P, U, U0, Uo, k);
   } else {
+    switch (L.mix2) {
+      case M3_SPW:
+        return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC, M3_SPW>, 2, X2, Y2, 2>(
+            L, P, U, U0, Uo, k);
+      case M3_DSP_SIMT:
+        return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC, M3_DSP_SIMT>, 2, X2, Y2, 2>(
+            L, P, U, U0, Uo, k);
+      case M3_DSP_BF16:
+        return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC, M3_DSP_BF16>, 2, X2, Y2, 2>(
+            L, P, U, U0, Uo, k);
+      case M3_DSP_FP16:
+        return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC, M3_DSP_FP16>, 2, X2, Y2, 2>(
+            L, P, U, U0, Uo, k);
+      default:
+        break;
+    }
     if (L.G == 3)
       return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 3>(L, P, U, U0, Uo,
                                                                                  k);

@@ -383,6 +383,89 @@ __device__ __forceinline__ float sel4(int i, float a, float b, float c, float d)
   return i == 0 ? a : i == 1 ? b : i == 2 ? c : d;
 }
 
+// reconstruction arithmetic of the FP32 jump-form faces on the FP64 state
+// (3D faces, and the 2D TeCNO variants other than the fused kepes_tecno4_roe
+// stage): SP-WENO, or DSP-WENO with the network as FP32 SIMT or one
+// v_mfma_f32_32x32x16 per layer
+enum { M3_NONE = 0, M3_SPW = 1, M3_DSP_SIMT = 2, M3_DSP_BF16 = 3, M3_DSP_FP16 = 4 };
+
+template <int NC>
+__device__ __forceinline__ float selc(int i, const float* a) {
+  float r = a[NC - 1];
+#pragma unroll
+  for (int k = NC - 2; k >= 0; --k) r = (i == k) ? a[k] : r;
+  return r;
+}
+
+// Reconstructed jumps <<Z_i>> of NC characteristic components from their
+// three FP32 cell jumps d0, d1, d2 and centre values z1: SP-WENO's jump
+// (limiter.h:193-252) or DSP-WENO's (neural_network.cc:16-77, 320-325,
+// zero where |d1| < 1e-15 as reconstruct.cc:152).  The matrix-core network
+// needs every lane of the wave (all lanes call this together).  With
+// sign_fallback a network jump whose sign disagrees with the cell jump is
+// replaced by SP-WENO's; nfb counts the replacements.
+template <int MIX, int NC>
+__device__ __forceinline__ void jumps_fp32(const Phys<double>& P, const float* d0,
+                                           const float* d1, const float* d2, const float* z1,
+                                           float* jmp, unsigned& nfb) {
+#pragma unroll
+  for (int i = 0; i < NC; ++i) jmp[i] = 0.0f;
+  if constexpr (MIX == M3_SPW) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+      jmp[i] = fast::spw_jump(d0[i], d1[i], d2[i], z1[i], z1[i] + d1[i], P.corr);
+  } else {
+    unsigned bad = 0;
+#ifndef TECNO_HOST_BUILD
+    [[maybe_unused]] mfma::Net5<MIX == M3_DSP_FP16> net;
+    if constexpr (MIX != M3_DSP_SIMT) net.load(P.mfma_w[MIX == M3_DSP_FP16 ? 1 : 0], threadIdx.x & 63);
+#endif
+#pragma unroll 1
+    for (int i = 0; i < NC; ++i) {
+      const float e1 = selc<NC>(i, d1);
+      const bool skip = !(fabsf(e1) >= 1e-15f);
+#ifndef TECNO_HOST_BUILD
+      if (__all(skip)) continue;  // zero centre jump on every lane: jump 0
+#endif
+      const float e0 = selc<NC>(i, d0), e2 = selc<NC>(i, d2), y1 = selc<NC>(i, z1);
+      const float y2 = y1 + e1, y0 = y1 - e0, y3 = y2 + e2;
+      const float maxabs = fmaxf(fmaxf(fabsf(y0), fabsf(y1)), fmaxf(fabsf(y2), fabsf(y3)));
+      float st[5], tm, tp, maxd;
+      fast::features_nv(e0, e1, e2, maxabs, st, tm, tp, maxd);
+      float j;
+#ifndef TECNO_HOST_BUILD
+      if constexpr (MIX != M3_DSP_SIMT) {
+        // a skipped lane feeds zeros: its partner lane in the matrix column
+        // must not see non-finite ratios
+#pragma unroll
+        for (int k = 0; k < 5; ++k) st[k] = skip ? 0.0f : st[k];
+        float lg[5];
+        net.eval(st, lg);
+        j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
+      } else
+#endif
+      {
+        fast::body5(P.net32, st);
+        j = fast::sign_round(fast::head_combine(st, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
+      }
+      if (skip) j = 0.0f;
+      bad |= (j * e1 < 0.0f) ? (1u << i) : 0u;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) jmp[k] = (k == i) ? j : jmp[k];
+    }
+    if (!P.sign_fallback) bad = 0;
+    while (bad) {
+      const int i = __builtin_ctz(bad);
+      bad &= bad - 1;
+      const float e1 = selc<NC>(i, d1), y1 = selc<NC>(i, z1);
+      const float j = fast::spw_jump(selc<NC>(i, d0), e1, selc<NC>(i, d2), y1, y1 + e1, 0);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) jmp[k] = (k == i) ? j : jmp[k];
+      ++nfb;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- FP64 helpers
 #ifdef TECNO_HOST_BUILD
 __device__ __forceinline__ double rcp64(double x) { return 1.0 / x; }

@@ -251,8 +251,20 @@ __device__ __forceinline__ void roe2(const Phys<T>& P, T rl, T nl, T tl, T pl, T
 enum { FV_KEPES4_ROE = 0, FV_GENERIC = 1 };
 enum { RV_SPWENO = 0, RV_DSP = 1, RV_GENERIC = 2 };
 
-template <typename T, int G, int FV, int RV>
+// FP32 jumps of NC characteristic components (defined in fv_mixed.hpp)
+template <int MIX, int NC>
+__device__ __forceinline__ void jumps_fp32(const Phys<double>& P, const float* d0,
+                                           const float* d1, const float* d2, const float* z1,
+                                           float* jmp, unsigned& nfb);
+
+// MIX (M3_* of fv_mixed.hpp) != 0: the TeCNO face with its four
+// characteristic jumps reconstructed in FP32 jump form (jumps_fp32) on the
+// FP64 state -- the FP32 path of the TeCNO variants other than
+// kepes_tecno4_roe, whose fused stage is Euler2Mixed
+template <typename T, int G, int FV, int RV, int MIX = 0>
 struct Euler2Ph {
+  static_assert(MIX == 0 || (sizeof(T) == 8 && G == 2 && FV == FV_GENERIC && RV == RV_GENERIC),
+                "the FP32 jumps run on the FP64 state of the generic face");
   using TS = T;
   // waves per SIMD the register budget is cut to: three for DSP-WENO (its
   // network chain needs the latency hiding; KH 2048^2 stage 1.29 ms vs 1.43
@@ -462,7 +474,23 @@ struct Euler2Ph {
       Z4[l] = ((Rr4 * f + x4) + y4) + RE4 * lv;
     }
     T dZ0, dZ1, dZ2, dZ4;
-    {
+    if constexpr (MIX != 0) {
+      // projected jumps of neighbouring cells (FP64 differences), FP32 jumps
+      float d0[4], d1[4], d2[4], z1[4], jmp[4];
+      const T* Zc[4] = {Z0, Z1, Z2, Z4};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        d0[i] = (float)(Zc[i][L] - Zc[i][L - 1]);
+        d1[i] = (float)(Zc[i][R] - Zc[i][L]);
+        d2[i] = (float)(Zc[i][R + 1] - Zc[i][R]);
+        z1[i] = (float)Zc[i][L];
+      }
+      jumps_fp32<MIX, 4>(P, d0, d1, d2, z1, jmp, nfb);
+      dZ0 = jmp[0];
+      dZ1 = jmp[1];
+      dZ2 = jmp[2];
+      dZ4 = jmp[3];
+    } else {
       T l_, r_;
       if (RV == RV_SPWENO) {
         sp_weno(Z0[L - 1], Z0[L], Z0[R], Z0[R + 1], P.corr, l_, r_); dZ0 = r_ - l_;

@@ -21,8 +21,9 @@
 // A 2D or 1D problem is solved on its active axes; files repeat the
 // collapsed layers the reference stores (Nz = 2 for "2D"), and the error
 // norms count them, as the reference's sums do.
-// Options after the file: --fp32 (mixed-precision path), --mlp
-// fp64|fp32|bf16|fp16, --fix-zmom, --fix-edges, --sign-fallback, --quiet.
+// Options after the file: --fp32 (FP32 path), --fp32-state64 (scalar: FP32
+// reconstruction on an FP64 state), --mlp fp64|fp32|bf16|fp16, --fix-zmom,
+// --fix-edges, --fix-visc, --sign-fallback, --quiet.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -261,12 +262,15 @@ void write_ooc(const Param& p, const std::vector<Err>& E, int which, const std::
 
 int main(int argc, char** argv) {
   std::string file;
-  int precision = TECNO_FP64, mlp = -1, fix_zmom = 0, sign_fallback = 0, fix_edges = 0;
+  int precision = TECNO_FP64, mlp = -1, fix_zmom = 0, sign_fallback = 0, fix_edges = 0,
+      fix_visc = 0;
   bool quiet = false, dry_run = false;  // dry run: parse, evaluate the IC, touch no GPU
   for (int a = 1; a < argc; ++a) {
     const std::string s = argv[a];
     if (s == "-i" && a + 1 < argc) file = argv[++a];
     else if (s == "--fp32") precision = TECNO_FP32;
+    else if (s == "--fp32-state64") precision = TECNO_FP32_STATE64;
+    else if (s == "--fix-visc") fix_visc = 1;
     else if (s == "--fix-zmom") fix_zmom = 1;
     else if (s == "--fix-edges") fix_edges = 1;
     else if (s == "--sign-fallback") sign_fallback = 1;
@@ -282,8 +286,8 @@ int main(int argc, char** argv) {
       }
     } else if (file.empty() && s[0] != '-') file = s;
     else {
-      std::cerr << "usage: tecno -i param.in [--fp32] [--mlp fp64|fp32|bf16|fp16] [--fix-zmom] [--fix-edges] "
-                   "[--sign-fallback] [--quiet]\n";
+      std::cerr << "usage: tecno -i param.in [--fp32 | --fp32-state64] [--mlp fp64|fp32|bf16|fp16] "
+                   "[--fix-zmom] [--fix-edges] [--fix-visc] [--sign-fallback] [--quiet]\n";
       return 2;
     }
   }
@@ -354,6 +358,7 @@ int main(int argc, char** argv) {
       cfg.fix_zmom = fix_zmom;
       cfg.sign_fallback = sign_fallback;
       cfg.fix_edges = fix_edges;
+      cfg.fix_visc = fix_visc;
       cfg.mlp_dtype = mlp >= 0 ? mlp : (precision == TECNO_FP32 ? TECNO_MLP_FP32 : TECNO_MLP_FP64);
       r.ax = r.Nx;
       r.ay = r.dims >= 2 ? r.Ny : 1;

@@ -25,6 +25,7 @@ namespace fast {
 __device__ __forceinline__ float rcp(float x) { return 1.0f / x; }
 __device__ __forceinline__ float log(float x) { return std::log(x); }
 __device__ __forceinline__ float exp(float x) { return std::exp(x); }
+__device__ __forceinline__ float exp2(float x) { return std::exp2(x); }
 __device__ __forceinline__ float sqrt(float x) { return std::sqrt(x); }
 #else
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -34,6 +35,7 @@ __device__ __forceinline__ float log(float x) {
 __device__ __forceinline__ float exp(float x) {
   return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
 }
+__device__ __forceinline__ float exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 #endif
 __device__ __forceinline__ float div(float a, float b) { return a * rcp(b); }
@@ -281,9 +283,12 @@ __device__ __forceinline__ float head_combine(const float lg[5], float tm, float
   float mx = 0.0f;
 #pragma unroll
   for (int i = 0; i < 5; ++i) mx = fmaxf(mx, lg[i]);
+  // exp(l - mx) as 2^(l log2 e - mx log2 e): one fma per weight
+  constexpr float L2E = 1.44269504088896341f;
+  const float mxl = mx * L2E;
   float s[5];
 #pragma unroll
-  for (int i = 0; i < 5; ++i) s[i] = exp(lg[i] - mx);
+  for (int i = 0; i < 5; ++i) s[i] = exp2(fmaf(lg[i], L2E, -mxl));
   float C1, C2;
   vertex_combine(tm, tp, maxd, s, C1, C2);
   return 0.5f * ((0.25f - 2.0f * C1) * (d1 - d0) + (0.25f - 2.0f * C2) * (d1 - d2));

@@ -26,6 +26,10 @@ namespace tecno {
 // it, entropy variables (fv.cc:421-431)
 enum { E3_RHO = 0, E3_U, E3_V, E3_W, E3_P, E3_BK, E3_VF, E3_V1, E3_V2, E3_V3, E3_VL, E3_NREC };
 
+#ifndef TECNO_STAGE3_WAVES
+#define TECNO_STAGE3_WAVES 2
+#endif
+
 // 3D face: the five characteristic jumps in FP32 jump form.  The
 // differences of the FP64 entropy variables of neighbouring cells are
 // projected on the scaled eigenvectors Rm in FP64 (a projected jump can be a
@@ -248,6 +252,13 @@ struct Euler3Ph {
   using TF = T;
   using TP = T;
   static constexpr int NV = 5, NREC = E3_NREC;
+  static constexpr bool FR_SELECT = MIX != M3_NONE;
+  // waves per SIMD the register budget is cut to
+#ifdef TECNO_STAGE3_MIX_WAVES
+  static constexpr int MIN_WAVES3 = MIX == M3_NONE ? TECNO_STAGE3_WAVES : TECNO_STAGE3_MIX_WAVES;
+#else
+  static constexpr int MIN_WAVES3 = TECNO_STAGE3_WAVES;
+#endif
 
   __device__ static Pr3<T> prim(const T* ld, int NB, int c) {
     Pr3<T> q;
@@ -649,7 +660,19 @@ struct Tile3 {
       // lanes past the end evaluate the last face again (the whole wave
       // takes part in the matrix-core network); their count is not kept
       unsigned nf = 0;
-      PH::face(P, ld, NB, base, stride, ax, Fr[rd], nf);
+      if constexpr (PH::FR_SELECT) {
+        // constant indices into Fr: the compiler keeps this loop rolled for
+        // the FP32-network faces, and Fr[rd] would live in scratch memory
+        // (TGV 128^3 FP32 DSP-WENO +7%; the FP64 faces are faster without)
+        T F[NV];
+        PH::face(P, ld, NB, base, stride, ax, F, nf);
+#pragma unroll
+        for (int r2 = 0; r2 < ROUNDS; ++r2)
+#pragma unroll
+          for (int v = 0; v < NV; ++v) Fr[r2][v] = (r2 == rd) ? F[v] : Fr[r2][v];
+      } else {
+        PH::face(P, ld, NB, base, stride, ax, Fr[rd], nf);
+      }
       if (rd * 256 + tid < NF) nfb += nf;
     }
   }
@@ -734,11 +757,8 @@ struct Tile3 {
 };
 
 #ifndef TECNO_HOST_BUILD
-#ifndef TECNO_STAGE3_WAVES
-#define TECNO_STAGE3_WAVES 2
-#endif
 template <class PH, int TX, int TY, int TZ, int G, bool VISC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TECNO_STAGE3_WAVES)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PH::MIN_WAVES3)))
 void stage3_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDev g) {
   using TL = Tile3<PH, TX, TY, TZ, G, VISC>;
   using T = typename PH::TS;

@@ -253,6 +253,11 @@ struct Euler3Ph {
   using TP = T;
   static constexpr int NV = 5, NREC = E3_NREC;
   static constexpr bool FR_SELECT = MIX != M3_NONE;
+  // The FP64 kernel of FAM 0 runs at the 256-VGPR limit of two waves per
+  // SIMD: with the XCD-aware tile order and the stage-dependent state reads
+  // its compiler output spills 1.4 KB per lane (TGV 128^3 stage twice as
+  // slow), so it keeps the launch order and reads U and U^n in every stage.
+  static constexpr bool REG_SLACK = !(FAM == 0 && MIX == M3_NONE);
   // waves per SIMD the register budget is cut to
 #ifdef TECNO_STAGE3_MIX_WAVES
   static constexpr int MIN_WAVES3 = MIX == M3_NONE ? TECNO_STAGE3_WAVES : TECNO_STAGE3_MIX_WAVES;
@@ -713,6 +718,11 @@ struct Tile3 {
     const int64_t N = g.N;
     const double factor = dt / P.vol_d;
     const T a = T(A.a), b = T(A.b), fb = T(factor * A.c);
+    // U is read where the stage weighs it (b != 0), U^n where a != 0 or for the
+    // classical RK4 forms: SSPRK3's first stage and rk1 (a = 0) skip U^n,
+    // jameson_rk4 (b = 0) skips U
+    const bool rd_u = !PH::REG_SLACK || (A.r2 == 0 && A.b != 0.0);
+    const bool rd_u0 = !PH::REG_SLACK || A.r2 != 0 || A.a != 0.0;
     for (int c = tid; c < NCELL; c += 256) {
       const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
       const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
@@ -744,8 +754,8 @@ struct Tile3 {
                      lnode[(8 + v - 1) * NN + nd] * sz;
               }
         }
-        const T u = A.U[v * N + idx];
-        const T u0 = A.U0[v * N + idx];
+        const T u = rd_u ? A.U[v * N + idx] : T(0);
+        const T u0 = rd_u0 ? A.U0[v * N + idx] : T(0);
         A.Uout[v * N + idx] = rk_update<NV>(A, v * N + idx, u, u0, r, a, b, fb, factor);
       }
       if (A.last) {
@@ -766,7 +776,9 @@ void stage3_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDe
   __shared__ double red[4];
   if (!A.ctl->active) return;
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * TZ;
+  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  if constexpr (PH::REG_SLACK) tile_of_block(bx, by, bz);
+  const int x0 = bx * TX, y0 = by * TY, z0 = bz * TZ;
   T* ld = (T*)lds;
   TL::load(A, P, g, ld, tid, x0, y0, z0);
   __syncthreads();

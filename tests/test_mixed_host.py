@@ -65,6 +65,22 @@ int main() {
     std::fprintf(f, "%.17g %.17g %.17g\n", a, b, logavg_ln(a, b, std::log(a), std::log(b)));
   }
   std::fclose(f);
+  // record logarithm: largest error in ulp of std::log over 24 decades, near 1,
+  // at powers of two and at the boundary of the mantissa range; NaN for x <= 0
+  double ulp_max = 0.0, abs_near1 = 0.0;
+  std::uniform_real_distribution<double> E(-40.0, 40.0), W(-1e-3, 1e-3);
+  for (int it = 0; it < 400000; ++it) {
+    double x = std::exp(E(h));
+    if (it % 4 == 1) x = 1.0 + W(h);
+    if (it % 64 == 2) x = std::ldexp(1.0, it % 200 - 100);
+    if (it % 64 == 3) x = std::ldexp(0.70710678118654752 * (1.0 + W(h) * 1e-9), it % 40 - 20);
+    const double a = log64(x), b = std::log(x);
+    if (b != 0.0) ulp_max = std::fmax(ulp_max, std::fabs(a - b) / (std::nextafter(std::fabs(b), INFINITY) - std::fabs(b)));
+    else ulp_max = std::fmax(ulp_max, std::fabs(a) * 1e300);
+    if (std::fabs(x - 1.0) < 1e-3) abs_near1 = std::fmax(abs_near1, std::fabs(a - b));
+  }
+  std::printf("log64 %.3f %.3e %d %d %d\n", ulp_max, abs_near1, (int)std::isnan(log64(0.0)),
+              (int)std::isnan(log64(-2.0)), (int)std::isnan(log64(NAN)));
   return 0;
 }
 """
@@ -107,3 +123,15 @@ def test_logavg_from_cell_logs_matches_oracle(harness_output):
     ref = orc.logavg(vals[:, 0], vals[:, 1])
     rel = np.abs(vals[:, 2] - ref) / np.abs(ref)
     assert rel.max() <= 4e-15
+
+
+def test_record_log_within_few_ulp(harness_output):
+    """log64 (the per-cell logarithm of the mixed path's records) against the
+    C library's log: at most 2 ulp anywhere, 1e-18 absolute near 1, NaN for
+    x <= 0 and NaN input.  (The 2-ulp cases are x just above 1, where
+    m + 1 rounds.)"""
+    out, _ = harness_output
+    f = out.split("log64")[1].split()
+    assert float(f[0]) <= 2.25
+    assert float(f[1]) <= 1e-18
+    assert f[2:5] == ["1", "1", "1"]

@@ -721,8 +721,7 @@ struct Tile3 {
     // U is read where the stage weighs it (b != 0), U^n where a != 0 or for the
     // classical RK4 forms: SSPRK3's first stage and rk1 (a = 0) skip U^n,
     // jameson_rk4 (b = 0) skips U
-    const bool rd_u = !PH::REG_SLACK || (A.r2 == 0 && A.b != 0.0);
-    const bool rd_u0 = !PH::REG_SLACK || A.r2 != 0 || A.a != 0.0;
+    const bool rd_u = A.r2 == 0 && A.b != 0.0, rd_u0 = A.r2 != 0 || A.a != 0.0;
     for (int c = tid; c < NCELL; c += 256) {
       const int lx = c % TX, ly = (c / TX) % TY, lz = c / (TX * TY);
       const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
@@ -731,6 +730,17 @@ struct Tile3 {
       const int fxl = (lz * TY + ly) * (TX + 1) + lx;
       const int fyl = NFX + (lz * (TY + 1) + ly) * TX + lx;
       const int fzl = NFX + NFY + (lz * TY + ly) * TX + lx;
+      // every state read of the cell is issued before its first store (Uout
+      // may be U^n: a read after a store would wait for it); the FP64 FAM 0
+      // kernel keeps its reads next to their use (register budget)
+      T u[NV], u0[NV], un[NV];
+      if constexpr (PH::REG_SLACK) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          u[v] = rd_u ? A.U[v * N + idx] : T(0);
+          u0[v] = rd_u0 ? A.U0[v * N + idx] : T(0);
+        }
+      }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         T r = -lf[v * NF + fxl];
@@ -754,13 +764,17 @@ struct Tile3 {
                      lnode[(8 + v - 1) * NN + nd] * sz;
               }
         }
-        const T u = rd_u ? A.U[v * N + idx] : T(0);
-        const T u0 = rd_u0 ? A.U0[v * N + idx] : T(0);
-        A.Uout[v * N + idx] = rk_update<NV>(A, v * N + idx, u, u0, r, a, b, fb, factor);
+        if constexpr (!PH::REG_SLACK) {
+          u[v] = A.U[v * N + idx];
+          u0[v] = A.U0[v * N + idx];
+        }
+        un[v] = rk_update<NV>(A, v * N + idx, u[v], u0[v], r, a, b, fb, factor);
+        A.Uout[v * N + idx] = un[v];
       }
       if (A.last) {
-        smax = m_max(smax, PH::speed(P, A.Uout, N, idx));
-        nbad += PH::bad(A.Uout, N, idx) ? 1u : 0u;
+        // from the registers of the new state (a one-cell view)
+        smax = m_max(smax, PH::speed(P, un, 1, 0));
+        nbad += PH::bad(un, 1, 0) ? 1u : 0u;
       }
     }
   }

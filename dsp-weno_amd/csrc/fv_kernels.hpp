@@ -276,6 +276,15 @@ struct Tile {
       const int64_t idx = (int64_t)gy * g.n[0] + gx;
       const int fxl = ly * (TX + 1) + lx;
       const int fyl = SH::NFX + ly * TX + lx;
+      // every state read of the cell is issued before its first store: Uout
+      // may be U^n (the last SSPRK3 stage writes in place), so a read after a
+      // store would wait for it, one memory round trip per variable
+      TS u[NV], u0[NV], un[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        u[v] = rd_u ? A.U[v * N + idx] : TS(0);
+        u0[v] = rd_u0 ? A.U0[v * N + idx] : TS(0);
+      }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         TS r = -TS(lds[v * NF + fxl]);
@@ -284,13 +293,14 @@ struct Tile {
           r -= TS(lds[v * NF + fyl]);
           r += TS(lds[v * NF + fyl + TX]);
         }
-        const TS u = rd_u ? A.U[v * N + idx] : TS(0);
-        const TS u0 = rd_u0 ? A.U0[v * N + idx] : TS(0);
-        A.Uout[v * N + idx] = rk_update<NV>(A, v * N + idx, u, u0, r, a, b, fb, factor);
+        un[v] = rk_update<NV>(A, v * N + idx, u[v], u0[v], r, a, b, fb, factor);
+        A.Uout[v * N + idx] = un[v];
       }
       if (A.last) {
-        smax = m_max(smax, PH::speed(P, A.Uout, N, idx));
-        nbad += PH::bad(A.Uout, N, idx) ? 1u : 0u;
+        // wave speed and admissibility of the new state from its registers
+        // (a one-cell view: element v of un is variable v)
+        smax = m_max(smax, PH::speed(P, un, 1, 0));
+        nbad += PH::bad(un, 1, 0) ? 1u : 0u;
       }
     }
   }

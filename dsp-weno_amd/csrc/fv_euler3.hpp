@@ -89,14 +89,29 @@ __device__ __forceinline__ T vdot3(const T a[3], const T b[3]) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
+// FAST: the FP32 reconstruction paths' FP64 flux, measured against the FP64
+// path within a tolerance, takes the refined reciprocal for divisions and
+// logavg_fast for the logarithmic means (a few ulp from the reference form,
+// which the FP64 path keeps)
+template <bool FAST, typename T>
+__device__ __forceinline__ T lavg(T a, T b) {
+  if constexpr (FAST) return logavg_fast(a, b);
+  else return logavg(a, b);
+}
+template <bool FAST, typename T>
+__device__ __forceinline__ T qdiv(T a, T b) {
+  if constexpr (FAST) return a * rcp64(b);
+  else return a / b;
+}
+
 // kepec_flux kepec.cc:10-48
-template <typename T>
+template <bool FAST = false, typename T>
 __device__ __forceinline__ void kepec3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,
                                        const T n[3], T area, T F[5]) {
-  const T rho = logavg(l.rho, r.rho);
+  const T rho = lavg<FAST>(l.rho, r.rho);
   const T vel2 = T(0.5) * (vdot3(l.u, l.u) + vdot3(r.u, r.u));
-  const T beta = logavg(l.bk, r.bk);
-  const T p = T(0.5) * (l.rho + r.rho) / (l.bk + r.bk);
+  const T beta = lavg<FAST>(l.bk, r.bk);
+  const T p = qdiv<FAST>(T(0.5) * (l.rho + r.rho), l.bk + r.bk);
   T vel[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) vel[d] = l.u[d] * T(0.5) + r.u[d] * T(0.5);
@@ -105,7 +120,7 @@ __device__ __forceinline__ void kepec3(const Phys<T>& P, const Pr3<T>& l, const 
   T mom[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) mom[d] = n[d] * p + vel[d] * mass;
-  const T en = T(0.5) * (T(1) / ((P.gamma - T(1)) * beta) - vel2) * mass + vdot3(mom, vel);
+  const T en = T(0.5) * (qdiv<FAST>(T(1), (P.gamma - T(1)) * beta) - vel2) * mass + vdot3(mom, vel);
   F[0] = mass * area;
   F[1] = mom[0] * area;
   F[2] = mom[1] * area;
@@ -114,23 +129,25 @@ __device__ __forceinline__ void kepec3(const Phys<T>& P, const Pr3<T>& l, const 
 }
 
 // roe_ec_flux roe_ec.cc:10-69
-template <typename T>
+template <bool FAST = false, typename T>
 __device__ __forceinline__ void roeec3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,
                                        const T n[3], T area, T F[5]) {
-  const T bl = m_sqrt(l.rho / l.p), br = m_sqrt(r.rho / r.p);
+  const T bl = m_sqrt(qdiv<FAST>(l.rho, l.p)), br = m_sqrt(qdiv<FAST>(r.rho, r.p));
   const T z1b = T(0.5) * (bl + br);
   T zvb[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) zvb[d] = l.u[d] * (T(0.5) * bl) + r.u[d] * (T(0.5) * br);
   const T z5b = T(0.5) * (bl * l.p + br * r.p);
   const T zvn = vdot3(zvb, n);
-  const T z1l = logavg(bl, br);
-  const T z5l = logavg(bl * l.p, br * r.p);
+  const T z1l = lavg<FAST>(bl, br);
+  const T z5l = lavg<FAST>(bl * l.p, br * r.p);
   const T mass = z5l * zvn;
   T mom[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) mom[d] = n[d] * (z5b / z1b) + zvb[d] * (mass / z1b);
-  const T en = T(0.5) * ((P.gamma + T(1)) / (P.gamma - T(1)) * mass / z1l + vdot3(mom, zvb)) / z1b;
+  for (int d = 0; d < 3; ++d) mom[d] = n[d] * qdiv<FAST>(z5b, z1b) + zvb[d] * qdiv<FAST>(mass, z1b);
+  const T en = qdiv<FAST>(T(0.5) * (qdiv<FAST>((P.gamma + T(1)) / (P.gamma - T(1)) * mass, z1l) +
+                                    vdot3(mom, zvb)),
+                          z1b);
   F[0] = mass * area;
   F[1] = mom[0] * area;
   F[2] = mom[1] * area;
@@ -139,7 +156,7 @@ __device__ __forceinline__ void roeec3(const Phys<T>& P, const Pr3<T>& l, const 
 }
 
 // kep_flux kep.cc:10-44 with Enthalpy fv.cc:565-569
-template <typename T>
+template <bool FAST = false, typename T>
 __device__ __forceinline__ void kep3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,
                                      const T n[3], T area, T F[5]) {
   const T rho = T(0.5) * (l.rho + r.rho);
@@ -147,9 +164,9 @@ __device__ __forceinline__ void kep3(const Phys<T>& P, const Pr3<T>& l, const Pr
   T vel[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) vel[d] = l.u[d] * T(0.5) + r.u[d] * T(0.5);
-  const T al = m_sqrt(P.gamma * l.p / l.rho), ar = m_sqrt(P.gamma * r.p / r.rho);
-  const T Hl = al * al / (P.gamma - T(1)) + T(0.5) * vdot3(l.u, l.u);
-  const T Hr = ar * ar / (P.gamma - T(1)) + T(0.5) * vdot3(r.u, r.u);
+  const T al = m_sqrt(qdiv<FAST>(P.gamma * l.p, l.rho)), ar = m_sqrt(qdiv<FAST>(P.gamma * r.p, r.rho));
+  const T Hl = qdiv<FAST>(al * al, P.gamma - T(1)) + T(0.5) * vdot3(l.u, l.u);
+  const T Hr = qdiv<FAST>(ar * ar, P.gamma - T(1)) + T(0.5) * vdot3(r.u, r.u);
   const T H = T(0.5) * (Hl + Hr);
   const T vn = vdot3(vel, n);
   const T mass = rho * vn;
@@ -281,24 +298,30 @@ struct Euler3Ph {
   __device__ static void load(const Phys<T>& P, const T* U, int64_t N, int64_t idx, int flip_axis,
                               T* ld, int NB, int c) {
     const T rho = U[idx], E = U[4 * N + idx];
-    const T inv = T(1) / rho;
+    // the FP32 reconstruction path (MIX) is measured against the FP64 path
+    // within a tolerance, so its records take the refined reciprocal and
+    // log64 (2 ulp) where the FP64 path keeps the reference's divisions and
+    // the library logarithm
+    const T inv = MIX != M3_NONE ? rcp64(rho) : T(1) / rho;
     T u[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) u[d] = U[(1 + d) * N + idx] * inv;
     const T m2 = U[N + idx] * U[N + idx] + U[2 * N + idx] * U[2 * N + idx] +
                  U[3 * N + idx] * U[3 * N + idx];
-    const T p = (P.gamma - T(1)) * (E - T(0.5) * m2 / rho);
+    const T p = (P.gamma - T(1)) * (E - T(0.5) * (MIX != M3_NONE ? m2 * inv : m2 / rho));
 #pragma unroll
     for (int d = 0; d < 3; ++d)
       if (flip_axis & (1 << d)) u[d] = -u[d];
-    const T beta = rho / (T(2) * p);
-    const T s = m_log(p) - P.gamma * m_log(rho);
+    const T beta = MIX != M3_NONE ? T(0.5) * rho * rcp64(p) : rho / (T(2) * p);
+    const T s = MIX != M3_NONE ? log64(p) - P.gamma * log64(rho)
+                               : m_log(p) - P.gamma * m_log(rho);
     ld[E3_RHO * NB + c] = rho;
     ld[E3_U * NB + c] = u[0];
     ld[E3_V * NB + c] = u[1];
     ld[E3_W * NB + c] = u[2];
     ld[E3_P * NB + c] = p;
-    ld[E3_BK * NB + c] = T(0.5) / (P.R * (p / (P.R * rho)));
+    // 0.5 / (R T) with T = p / (R rho): beta itself on the FP32 path
+    ld[E3_BK * NB + c] = MIX != M3_NONE ? beta : T(0.5) / (P.R * (p / (P.R * rho)));
     ld[E3_VF * NB + c] = (P.gamma - s) / (P.gamma - T(1)) - vdot3(u, u) * beta;
     ld[E3_V1 * NB + c] = u[0] * (T(2) * beta);
     ld[E3_V2 * NB + c] = u[1] * (T(2) * beta);
@@ -323,13 +346,14 @@ struct Euler3Ph {
 #pragma unroll
     for (int l = 0; l < S; ++l) q[l] = prim(ld, NB, base + l * stride);
     const int fl = P.flux;
+    constexpr bool FAST = MIX != M3_NONE;
     auto ec = [&](int kind, const Pr3<T>& a, const Pr3<T>& b, T out[5]) {
       if (kind == 0)
-        kepec3(P, a, b, n, area, out);
+        kepec3<FAST>(P, a, b, n, area, out);
       else if (kind == 1)
-        roeec3(P, a, b, n, area, out);
+        roeec3<FAST>(P, a, b, n, area, out);
       else if (kind == 2 || FAM == 0)
-        kep3(P, a, b, n, area, out);
+        kep3<FAST>(P, a, b, n, area, out);
       else
         keps3(P, a, b, n, area, out);
     };
@@ -366,21 +390,21 @@ struct Euler3Ph {
     else if (fl == 105) ec4(1, F);
     else if (fl == 104) ec(1, q[L], q[R], F);
     else ec(2, q[L], q[R], F);  // 106 kep_tecno_roe
-    const T inv_area = T(1) / area;
+    const T inv_area = qdiv<FAST>(T(1), area);
 #pragma unroll
     for (int v = 0; v < 5; ++v) F[v] *= inv_area;
     // means of the centre pair, eigensystem (kepes_tecno4_roe.cc:39-113)
     const Pr3<T>& pl = q[L];
     const Pr3<T>& pr = q[R];
-    const T rho = logavg(pl.rho, pr.rho);
-    const T beta = logavg(pl.bk, pr.bk);
-    const T p = T(0.5) * (pl.rho + pr.rho) / (pl.bk + pr.bk);
+    const T rho = lavg<FAST>(pl.rho, pr.rho);
+    const T beta = lavg<FAST>(pl.bk, pr.bk);
+    const T p = qdiv<FAST>(T(0.5) * (pl.rho + pr.rho), pl.bk + pr.bk);
     T vel[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) vel[d] = pl.u[d] * T(0.5) + pr.u[d] * T(0.5);
     const T vn = vdot3(vel, n);
-    const T a = m_sqrt(T(0.5) * P.gamma / beta);
-    const T H = a * a / (P.gamma - T(1)) + T(0.5) * vdot3(vel, vel);
+    const T a = m_sqrt(qdiv<FAST>(T(0.5) * P.gamma, beta));
+    const T H = qdiv<FAST>(a * a, P.gamma - T(1)) + T(0.5) * vdot3(vel, vel);
     const T v1 = vel[0] * n[1] - vel[1] * n[0];
     const T v2 = vel[2] * n[0] - vel[0] * n[2];
     T Rm[5][5] = {{T(1), T(1), T(0), T(0), T(1)},
@@ -388,7 +412,7 @@ struct Euler3Ph {
                   {vel[1] - a * n[1], vel[1], -n[0], T(0), vel[1] + a * n[1]},
                   {vel[2] - a * n[2], vel[2], T(0), n[0], vel[2] + a * n[2]},
                   {H - a * vn, T(0.5) * vdot3(vel, vel), v1, v2, H + a * vn}};
-    const T const1 = m_sqrt(rho / P.gamma);
+    const T const1 = m_sqrt(qdiv<FAST>(rho, P.gamma));
     const T psq = m_sqrt(p);
     const T Sc[5] = {m_sqrt(T(0.5)) * const1, m_sqrt(P.gamma - T(1)) * const1, psq, psq,
                      m_sqrt(T(0.5)) * const1};
@@ -406,7 +430,8 @@ struct Euler3Ph {
       T ea = T(0);
       if (P.ec1_fix && fl != 106) ea = T(1.0 / 6.0);
       const T vnl = vdot3(pl.u, n), vnr = vdot3(pr.u, n);
-      const T al = m_sqrt(T(0.5) * P.gamma / pl.bk), ar = m_sqrt(T(0.5) * P.gamma / pr.bk);
+      const T al = m_sqrt(qdiv<FAST>(T(0.5) * P.gamma, pl.bk)),
+              ar = m_sqrt(qdiv<FAST>(T(0.5) * P.gamma, pr.bk));
       Lam[0] = (T(1) + T(0)) * m_abs(vn - a) + ea * m_abs((vnl - al) - (vnr - ar));
       Lam[1] = l2;
       Lam[2] = l2;

@@ -525,6 +525,23 @@ __device__ __forceinline__ double log64(double x) {
   return x > 0.0 ? res : __builtin_nan("");
 }
 
+// logarithmic mean (misc_func.h:14-32) for the FP64 flux of the FP32
+// reconstruction paths, which are measured against the FP64 path within a
+// tolerance: f = (b - a)/(b + a) (the reference's (xi - 1)/(xi + 1), xi =
+// b/a) from one refined reciprocal, the series' divisions by 3, 5, 7 as
+// products, and in the logarithm branch (a + b)/(2F) = (b - a)/ln(b/a) with
+// log64.  Within a few ulp of the reference form; 2 reciprocals where that
+// takes 7 divisions.
+__device__ __forceinline__ double logavg_fast(double a, double b) {
+  const double f = (b - a) * rcp64(b + a);
+  const double u = f * f;
+  if (u < 1.0e-2) {
+    const double F = 1.0 + u * (1.0 / 3.0) + (u * u) * (1.0 / 5.0) + (u * u * u) * (1.0 / 7.0);
+    return 0.5 * (a + b) * rcp64(F);
+  }
+  return (b - a) * rcp64(log64(b * rcp64(a)));
+}
+
 // logarithmic mean with the logarithms of the arguments known per cell
 // (misc_func.h:14-32): with f = (b-a)/(b+a) and u = f^2, the reference takes
 // (a+b) / (2 F(u)), F = 1 + u/3 + u^2/5 + u^3/7, for u < 1e-2 and otherwise

@@ -23,11 +23,32 @@
 namespace tecno {
 
 // record layout, double [E3_NREC][NB]: primitive, beta as kepec_flux computes
-// it, entropy variables (fv.cc:421-431)
+// it, entropy variables (fv.cc:421-431).  The FP32 reconstruction path keeps
+// the first E3_NREC_MIX fields only (its beta record is beta itself) and
+// forms V1..V3 and V5 from u and beta where they are read (e3_vent), so that
+// its records fit three workgroups per CU.
 enum { E3_RHO = 0, E3_U, E3_V, E3_W, E3_P, E3_BK, E3_VF, E3_V1, E3_V2, E3_V3, E3_VL, E3_NREC };
+constexpr int E3_NREC_MIX = E3_VF + 1;
+
+// entropy variable r (0..4) of record c: V1..V3 = u (2 beta) and V5 = -2 beta
+// are the products the full record stores, so both layouts give the same bits
+template <int MIX, typename T>
+__device__ __forceinline__ T e3_vent(const T* ld, int NB, int c, int r) {
+  if constexpr (MIX == 0) {
+    return ld[(E3_VF + r) * NB + c];
+  } else {
+    if (r == 0) return ld[E3_VF * NB + c];
+    const T b2 = T(2) * ld[E3_BK * NB + c];
+    if (r == 4) return -b2;
+    return ld[(E3_U + r - 1) * NB + c] * b2;
+  }
+}
 
 #ifndef TECNO_STAGE3_WAVES
 #define TECNO_STAGE3_WAVES 2
+#endif
+#ifndef TECNO_STAGE3_MIX_WAVES
+#define TECNO_STAGE3_MIX_WAVES 3
 #endif
 
 // 3D face: the five characteristic jumps in FP32 jump form.  The
@@ -46,7 +67,7 @@ __device__ __forceinline__ void jumps3_fp32(const Phys<double>& P, const double*
     for (int l = 0; l < 4; ++l) {
       const int cc = base + l * stride;
 #pragma unroll
-      for (int r = 0; r < 5; ++r) V[l][r] = ld[(E3_VF + r) * NB + cc];
+      for (int r = 0; r < 5; ++r) V[l][r] = e3_vent<MIX>(ld, NB, cc, r);
     }
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -268,19 +289,17 @@ struct Euler3Ph {
   using TS = T;
   using TF = T;
   using TP = T;
-  static constexpr int NV = 5, NREC = E3_NREC;
+  static constexpr int NV = 5, NREC = MIX != M3_NONE ? E3_NREC_MIX : E3_NREC;
   static constexpr bool FR_SELECT = MIX != M3_NONE;
   // The FP64 kernel of FAM 0 runs at the 256-VGPR limit of two waves per
   // SIMD: with the XCD-aware tile order and the stage-dependent state reads
   // its compiler output spills 1.4 KB per lane (TGV 128^3 stage twice as
   // slow), so it keeps the launch order and reads U and U^n in every stage.
   static constexpr bool REG_SLACK = !(FAM == 0 && MIX == M3_NONE);
-  // waves per SIMD the register budget is cut to
-#ifdef TECNO_STAGE3_MIX_WAVES
+  // waves per SIMD the register budget is cut to: the FP32 path's seven
+  // record fields (48 KB of LDS per 8x8x2 tile) let three workgroups share a
+  // CU, so its registers are cut to three waves per SIMD
   static constexpr int MIN_WAVES3 = MIX == M3_NONE ? TECNO_STAGE3_WAVES : TECNO_STAGE3_MIX_WAVES;
-#else
-  static constexpr int MIN_WAVES3 = TECNO_STAGE3_WAVES;
-#endif
 
   __device__ static Pr3<T> prim(const T* ld, int NB, int c) {
     Pr3<T> q;
@@ -323,10 +342,12 @@ struct Euler3Ph {
     // 0.5 / (R T) with T = p / (R rho): beta itself on the FP32 path
     ld[E3_BK * NB + c] = MIX != M3_NONE ? beta : T(0.5) / (P.R * (p / (P.R * rho)));
     ld[E3_VF * NB + c] = (P.gamma - s) / (P.gamma - T(1)) - vdot3(u, u) * beta;
-    ld[E3_V1 * NB + c] = u[0] * (T(2) * beta);
-    ld[E3_V2 * NB + c] = u[1] * (T(2) * beta);
-    ld[E3_V3 * NB + c] = u[2] * (T(2) * beta);
-    ld[E3_VL * NB + c] = T(-2) * beta;
+    if constexpr (MIX == M3_NONE) {
+      ld[E3_V1 * NB + c] = u[0] * (T(2) * beta);
+      ld[E3_V2 * NB + c] = u[1] * (T(2) * beta);
+      ld[E3_V3 * NB + c] = u[2] * (T(2) * beta);
+      ld[E3_VL * NB + c] = T(-2) * beta;
+    }
   }
 
   // FV::num_flux for one face from the 2G cells base + l*stride
@@ -489,8 +510,7 @@ struct Euler3Ph {
       T avg[5], Tx[5], Ty[5], Tz[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        const int r = E3_VF + k;
-        auto V = [&](int a, int b, int q) { return ld[r * NB + c[a][b][q]]; };
+        auto V = [&](int a, int b, int q) { return e3_vent<MIX>(ld, NB, c[a][b][q], k); };
         avg[k] = V(1, 1, 1) * f1 + V(0, 1, 1) * f1 + V(1, 0, 1) * f1 + V(1, 1, 0) * f1;
         avg[k] += V(0, 0, 1) * f1 + V(0, 1, 0) * f1 + V(1, 0, 0) * f1 + V(0, 0, 0) * f1;
         Tx[k] = V(1, 1, 1) * fx + V(1, 1, 0) * fx + V(1, 0, 1) * fx + V(1, 0, 0) * fx;

@@ -110,21 +110,6 @@ __device__ __forceinline__ T vdot3(const T a[3], const T b[3]) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-// FAST: the FP32 reconstruction paths' FP64 flux, measured against the FP64
-// path within a tolerance, takes the refined reciprocal for divisions and
-// logavg_fast for the logarithmic means (a few ulp from the reference form,
-// which the FP64 path keeps)
-template <bool FAST, typename T>
-__device__ __forceinline__ T lavg(T a, T b) {
-  if constexpr (FAST) return logavg_fast(a, b);
-  else return logavg(a, b);
-}
-template <bool FAST, typename T>
-__device__ __forceinline__ T qdiv(T a, T b) {
-  if constexpr (FAST) return a * rcp64(b);
-  else return a / b;
-}
-
 // kepec_flux kepec.cc:10-48
 template <bool FAST = false, typename T>
 __device__ __forceinline__ void kepec3(const Phys<T>& P, const Pr3<T>& l, const Pr3<T>& r,

@@ -472,75 +472,7 @@ __device__ __forceinline__ void jumps_fp32(const Phys<double>& P, const float* d
 }
 
 // ---------------------------------------------------------------- FP64 helpers
-#ifdef TECNO_HOST_BUILD
-__device__ __forceinline__ double rcp64(double x) { return 1.0 / x; }
-#else
-// v_rcp_f64 refined by two Newton steps (about 1 ulp)
-__device__ __forceinline__ double rcp64(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(fma(-x, r, 1.0), r, r);
-  r = fma(fma(-x, r, 1.0), r, r);
-  return r;
-}
-#endif
-
-// Natural logarithm of a double for the per-cell records of the mixed path.
-// x = m 2^e with m in [1/sqrt2, sqrt2) (frexp, then one doubling), and
-// ln m = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716:
-//   2 atanh(s) = 2s + 2s z (1/3 + z/5 + ... + z^8/19),  z = s^2 <= 0.0295;
-// the first omitted term, 2s z^10/21, is below 2^-55 of the sum.  s is the
-// residual-corrected quotient of the refined reciprocal, ln 2 is split into
-// a 42-bit head (e ln2_hi is exact for any exponent) and its tail.  About 35
-// instructions against about 75 for the library routine; within 2 ulp of the
-// C library's log (99.7% of arguments within 1 ulp; the host build of this
-// function is checked against std::log).
-// x <= 0 or NaN gives NaN, as the library routine does.
-__device__ __forceinline__ double log64(double x) {
-#ifdef TECNO_HOST_BUILD
-  int e0;
-  const double m0 = std::frexp(x, &e0);
-#else
-  const double m0 = __builtin_amdgcn_frexp_mant(x);
-  const int e0 = __builtin_amdgcn_frexp_exp(x);
-#endif
-  const bool lo = m0 < 0.70710678118654752;
-  const double m = lo ? m0 + m0 : m0;
-  const double e = (double)(lo ? e0 - 1 : e0);
-  const double f = m - 1.0, d = m + 1.0;
-  const double r = rcp64(d);
-  double s = f * r;
-  s = fma(fma(-s, d, f), r, s);
-  const double z = s * s, t = s + s;
-  double q = 1.0 / 19.0;
-  q = fma(q, z, 1.0 / 17.0);
-  q = fma(q, z, 1.0 / 15.0);
-  q = fma(q, z, 1.0 / 13.0);
-  q = fma(q, z, 1.0 / 11.0);
-  q = fma(q, z, 1.0 / 9.0);
-  q = fma(q, z, 1.0 / 7.0);
-  q = fma(q, z, 1.0 / 5.0);
-  q = fma(q, z, 1.0 / 3.0);
-  const double lnm = fma(t * z, q, t);
-  const double res = fma(e, 0x1.62e42fefa3800p-1, fma(e, 0x1.ef35793c76730p-45, lnm));
-  return x > 0.0 ? res : __builtin_nan("");
-}
-
-// logarithmic mean (misc_func.h:14-32) for the FP64 flux of the FP32
-// reconstruction paths, which are measured against the FP64 path within a
-// tolerance: f = (b - a)/(b + a) (the reference's (xi - 1)/(xi + 1), xi =
-// b/a) from one refined reciprocal, the series' divisions by 3, 5, 7 as
-// products, and in the logarithm branch (a + b)/(2F) = (b - a)/ln(b/a) with
-// log64.  Within a few ulp of the reference form; 2 reciprocals where that
-// takes 7 divisions.
-__device__ __forceinline__ double logavg_fast(double a, double b) {
-  const double f = (b - a) * rcp64(b + a);
-  const double u = f * f;
-  if (u < 1.0e-2) {
-    const double F = 1.0 + u * (1.0 / 3.0) + (u * u) * (1.0 / 5.0) + (u * u * u) * (1.0 / 7.0);
-    return 0.5 * (a + b) * rcp64(F);
-  }
-  return (b - a) * rcp64(log64(b * rcp64(a)));
-}
+// (rcp64, log64 and logavg_fast are in fv_device.hpp)
 
 // logarithmic mean with the logarithms of the arguments known per cell
 // (misc_func.h:14-32): with f = (b-a)/(b+a) and u = f^2, the reference takes

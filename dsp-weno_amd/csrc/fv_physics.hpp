@@ -93,23 +93,28 @@ struct ScalarPh {
 // kepec_flux computes it (enflo/src/kepec.cc:21-25).
 enum { E_RHO = 0, E_UX, E_UY, E_P, E_BK, E_VF, E_VX, E_VY, E_VL, E_NREC };
 
-template <typename T>
+// FAST: the FP32 reconstruction path's records (refined reciprocal, log64,
+// beta_k = beta), a few ulp from the reference's form that the FP64 path keeps
+template <bool FAST = false, typename T>
 __device__ __forceinline__ void euler2_record(const Phys<T>& P, T rho, T mx, T my, T E, int flipx,
                                               int flipy, T* rec) {
   // con_to_prim: enflo/src/fv.cc:388-400
-  T inv = T(1) / rho;
+  T inv = qdiv<FAST>(T(1), rho);
   T ux = mx * inv, uy = my * inv;
-  T p = P.gm1 * (E - T(0.5) * (mx * mx + my * my) / rho);
+  T p = P.gm1 * (E - (FAST ? T(0.5) * (mx * mx + my * my) * inv
+                           : T(0.5) * (mx * mx + my * my) / rho));
   if (flipx) ux = -ux;  // wall ghost: mirrored normal velocity (fv.cc:134-140)
   if (flipy) uy = -uy;
   // prim_to_ent: fv.cc:421-431, Entropy 557-560
-  T beta = rho / (T(2) * p);
-  T s = m_log(p) - P.gamma * m_log(rho);
+  T beta = FAST ? T(0.5) * rho * qdiv<FAST>(T(1), p) : rho / (T(2) * p);
+  T s;
+  if constexpr (FAST) s = log64(p) - P.gamma * log64(rho);
+  else s = m_log(p) - P.gamma * m_log(rho);
   rec[E_RHO] = rho;
   rec[E_UX] = ux;
   rec[E_UY] = uy;
   rec[E_P] = p;
-  rec[E_BK] = T(0.5) / (P.R * (p / (P.R * rho)));
+  rec[E_BK] = FAST ? beta : T(0.5) / (P.R * (p / (P.R * rho)));
   rec[E_VF] = (P.gamma - s) / P.gm1 - (ux * ux + uy * uy) * beta;
   rec[E_VX] = ux * (T(2) * beta);
   rec[E_VY] = uy * (T(2) * beta);
@@ -117,20 +122,20 @@ __device__ __forceinline__ void euler2_record(const Phys<T>& P, T rho, T mx, T m
 }
 
 // kepec_flux in the face frame (n = normal, t = tangential): kepec.cc:10-48
-template <typename T>
+template <bool FAST = false, typename T>
 __device__ __forceinline__ void kepec2(const Phys<T>& P, T rl, T nl, T tl, T bl, T rr, T nr, T tr,
                                        T br, T area, T F[4], T* rho_o = nullptr,
                                        T* beta_o = nullptr) {
-  T rho = logavg(rl, rr);
+  T rho = lavg<FAST>(rl, rr);
   T vel2 = T(0.5) * ((nl * nl + tl * tl) + (nr * nr + tr * tr));
-  T beta = logavg(bl, br);
-  T p = T(0.5) * (rl + rr) / (bl + br);
+  T beta = lavg<FAST>(bl, br);
+  T p = qdiv<FAST>(T(0.5) * (rl + rr), bl + br);
   T vn = nl * T(0.5) + nr * T(0.5);
   T vt = tl * T(0.5) + tr * T(0.5);
   T mass = rho * vn;
   T mn = p + vn * mass;
   T mt = vt * mass;
-  T en = T(0.5) * (T(1) / (P.gm1 * beta) - vel2) * mass + (mn * vn + mt * vt);
+  T en = T(0.5) * (qdiv<FAST>(T(1), P.gm1 * beta) - vel2) * mass + (mn * vn + mt * vt);
   F[0] = mass * area;
   F[1] = mn * area;
   F[2] = mt * area;
@@ -139,20 +144,20 @@ __device__ __forceinline__ void kepec2(const Phys<T>& P, T rl, T nl, T tl, T bl,
 }
 
 // Ismail-Roe EC flux (roe_ec.cc:10-69) in the face frame
-template <typename T>
+template <bool FAST = false, typename T>
 __device__ __forceinline__ void roeec2(const Phys<T>& P, T rl, T nl, T tl, T pl, T rr, T nr, T tr,
                                        T pr, T area, T F[4]) {
-  T bl = m_sqrt(rl / pl), br = m_sqrt(rr / pr);
+  T bl = m_sqrt(qdiv<FAST>(rl, pl)), br = m_sqrt(qdiv<FAST>(rr, pr));
   T z1b = T(0.5) * (bl + br);
   T zn = nl * (T(0.5) * bl) + nr * (T(0.5) * br);
   T zt = tl * (T(0.5) * bl) + tr * (T(0.5) * br);
   T z5b = T(0.5) * (bl * pl + br * pr);
-  T z1l = logavg(bl, br);
-  T z5l = logavg(bl * pl, br * pr);
+  T z1l = lavg<FAST>(bl, br);
+  T z5l = lavg<FAST>(bl * pl, br * pr);
   T mass = z5l * zn;
-  T mn = (z5b / z1b) + zn * (mass / z1b);
-  T mt = zt * (mass / z1b);
-  T en = T(0.5) * (P.gpgm * mass / z1l + (mn * zn + mt * zt)) / z1b;
+  T mn = qdiv<FAST>(z5b, z1b) + zn * qdiv<FAST>(mass, z1b);
+  T mt = zt * qdiv<FAST>(mass, z1b);
+  T en = qdiv<FAST>(T(0.5) * (qdiv<FAST>(P.gpgm * mass, z1l) + (mn * zn + mt * zt)), z1b);
   F[0] = mass * area;
   F[1] = mn * area;
   F[2] = mt * area;
@@ -160,16 +165,16 @@ __device__ __forceinline__ void roeec2(const Phys<T>& P, T rl, T nl, T tl, T pl,
 }
 
 // KEP flux (kep.cc:10-44), Enthalpy (fv.cc:565-569)
-template <typename T>
+template <bool FAST = false, typename T>
 __device__ __forceinline__ void kep2(const Phys<T>& P, T rl, T nl, T tl, T pl, T rr, T nr, T tr,
                                      T pr, T area, T F[4]) {
   T rho = T(0.5) * (rl + rr);
   T p = T(0.5) * (pl + pr);
   T vn = nl * T(0.5) + nr * T(0.5);
   T vt = tl * T(0.5) + tr * T(0.5);
-  T al = m_sqrt(P.gamma * pl / rl), ar = m_sqrt(P.gamma * pr / rr);
-  T Hl = al * al / P.gm1 + T(0.5) * (nl * nl + tl * tl);
-  T Hr = ar * ar / P.gm1 + T(0.5) * (nr * nr + tr * tr);
+  T al = m_sqrt(qdiv<FAST>(P.gamma * pl, rl)), ar = m_sqrt(qdiv<FAST>(P.gamma * pr, rr));
+  T Hl = qdiv<FAST>(al * al, P.gm1) + T(0.5) * (nl * nl + tl * tl);
+  T Hr = qdiv<FAST>(ar * ar, P.gm1) + T(0.5) * (nr * nr + tr * tr);
   T H = T(0.5) * (Hl + Hr);
   T mass = rho * vn;
   T mn = p + vn * mass;
@@ -277,6 +282,9 @@ struct Euler2Ph {
   using TF = T;
   using TP = T;
   static constexpr int NV = 4, NREC = E_NREC;
+  // the FP32 reconstruction path's FP64 records and fluxes take the
+  // refined reciprocal, log64 and logavg_fast
+  static constexpr bool FAST = MIX != 0;
   static constexpr size_t rec_bytes(int NB) { return (size_t)NREC * NB * sizeof(T); }
   // kepes_tecno4_roe: the distance-2 KEPEC fluxes F(0,2), F(1,3) of a face's
   // stencil are evaluated once per cell pair (with the face area, exactly as
@@ -287,7 +295,7 @@ struct Euler2Ph {
     const T* lds = (const T*)ldsv;
     const int rn = ax ? E_UY : E_UX, rt = ax ? E_UX : E_UY;
     T f0[4];
-    kepec2(P, lds[E_RHO * NB + ca], lds[rn * NB + ca], lds[rt * NB + ca], lds[E_BK * NB + ca],
+    kepec2<FAST>(P, lds[E_RHO * NB + ca], lds[rn * NB + ca], lds[rt * NB + ca], lds[E_BK * NB + ca],
            lds[E_RHO * NB + cb], lds[rn * NB + cb], lds[rt * NB + cb], lds[E_BK * NB + cb],
            P.area[ax], f0);
 #pragma unroll
@@ -296,7 +304,7 @@ struct Euler2Ph {
   __device__ static void load(const Phys<T>& P, const T* U, int64_t N, int64_t idx, int flipx,
                               int flipy, void* lds, int NB, int c) {
     T rec[NREC];
-    euler2_record(P, U[idx], U[N + idx], U[2 * N + idx], U[3 * N + idx], flipx, flipy, rec);
+    euler2_record<FAST>(P, U[idx], U[N + idx], U[2 * N + idx], U[3 * N + idx], flipx, flipy, rec);
 #pragma unroll
     for (int r = 0; r < NREC; ++r) ((T*)lds)[r * NB + c] = rec[r];
   }
@@ -366,14 +374,14 @@ struct Euler2Ph {
     // ---- entropy-conservative part
     if (fl == 101 || fl == 103 || fl == 121) {  // kepec4
       T f0[4];
-      kepec2(P, rho[L], un[L], ut[L], bk[L], rho[R], un[R], ut[R], bk[R], area, f0, &rhoh, &betah);
+      kepec2<FAST>(P, rho[L], un[L], ut[L], bk[L], rho[R], un[R], ut[R], bk[R], area, f0, &rhoh, &betah);
 #pragma unroll
       for (int v = 0; v < 4; ++v) Fe[v] = T(0) + f0[v] * T(4.0 / 3.0);
       if (G == 2 && pairs) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q];
       } else {
-        kepec2(P, rho[L - 1], un[L - 1], ut[L - 1], bk[L - 1], rho[R], un[R], ut[R], bk[R], area,
+        kepec2<FAST>(P, rho[L - 1], un[L - 1], ut[L - 1], bk[L - 1], rho[R], un[R], ut[R], bk[R], area,
                f0);
       }
 #pragma unroll
@@ -382,34 +390,34 @@ struct Euler2Ph {
 #pragma unroll
         for (int v = 0; v < 4; ++v) f0[v] = pairs[v * NP + q + ps];
       } else {
-        kepec2(P, rho[L], un[L], ut[L], bk[L], rho[R + 1], un[R + 1], ut[R + 1], bk[R + 1], area,
+        kepec2<FAST>(P, rho[L], un[L], ut[L], bk[L], rho[R + 1], un[R + 1], ut[R + 1], bk[R + 1], area,
                f0);
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) Fe[v] += f0[v] * T(-1.0 / 6.0);
     } else {
       // centre-pair logavg values are needed by every TeCNO variant
-      rhoh = logavg(rho[L], rho[R]);
-      betah = logavg(bk[L], bk[R]);
+      rhoh = lavg<FAST>(rho[L], rho[R]);
+      betah = lavg<FAST>(bk[L], bk[R]);
       T pl = lds[E_P * NB + base + L * stride], pr = lds[E_P * NB + base + R * stride];
       if (fl == 100 || fl == 102 || fl == 120) {
-        kepec2(P, rho[L], un[L], ut[L], bk[L], rho[R], un[R], ut[R], bk[R], area, Fe);
+        kepec2<FAST>(P, rho[L], un[L], ut[L], bk[L], rho[R], un[R], ut[R], bk[R], area, Fe);
       } else if (fl == 104 || fl == 122) {
-        roeec2(P, rho[L], un[L], ut[L], pl, rho[R], un[R], ut[R], pr, area, Fe);
+        roeec2<FAST>(P, rho[L], un[L], ut[L], pl, rho[R], un[R], ut[R], pr, area, Fe);
       } else if (fl == 105 || fl == 123) {
         T f0[4];
         T p0 = lds[E_P * NB + base + (L - 1) * stride], p3 = lds[E_P * NB + base + (R + 1) * stride];
-        roeec2(P, rho[L], un[L], ut[L], pl, rho[R], un[R], ut[R], pr, area, f0);
+        roeec2<FAST>(P, rho[L], un[L], ut[L], pl, rho[R], un[R], ut[R], pr, area, f0);
 #pragma unroll
         for (int v = 0; v < 4; ++v) Fe[v] = T(0) + f0[v] * T(4.0 / 3.0);
-        roeec2(P, rho[L - 1], un[L - 1], ut[L - 1], p0, rho[R], un[R], ut[R], pr, area, f0);
+        roeec2<FAST>(P, rho[L - 1], un[L - 1], ut[L - 1], p0, rho[R], un[R], ut[R], pr, area, f0);
 #pragma unroll
         for (int v = 0; v < 4; ++v) Fe[v] += f0[v] * T(-1.0 / 6.0);
-        roeec2(P, rho[L], un[L], ut[L], pl, rho[R + 1], un[R + 1], ut[R + 1], p3, area, f0);
+        roeec2<FAST>(P, rho[L], un[L], ut[L], pl, rho[R + 1], un[R + 1], ut[R + 1], p3, area, f0);
 #pragma unroll
         for (int v = 0; v < 4; ++v) Fe[v] += f0[v] * T(-1.0 / 6.0);
       } else {  // 106, 124: kep
-        kep2(P, rho[L], un[L], ut[L], pl, rho[R], un[R], ut[R], pr, area, Fe);
+        kep2<FAST>(P, rho[L], un[L], ut[L], pl, rho[R], un[R], ut[R], pr, area, Fe);
       }
     }
     if (pure_ec) {
@@ -419,18 +427,18 @@ struct Euler2Ph {
       F[3] = Fe[3];
       return;
     }
-    const T inv_area = T(1) / area;
+    const T inv_area = qdiv<FAST>(T(1), area);
 #pragma unroll
     for (int v = 0; v < 4; ++v) Fe[v] *= inv_area;
 
     // ---- entropy dissipation: R~ Lambda (Z+ - Z-)
-    const T p = T(0.5) * (rho[L] + rho[R]) / (bk[L] + bk[R]);
+    const T p = qdiv<FAST>(T(0.5) * (rho[L] + rho[R]), bk[L] + bk[R]);
     const T vn = un[L] * T(0.5) + un[R] * T(0.5);
     const T vt = ut[L] * T(0.5) + ut[R] * T(0.5);
-    const T a = m_sqrt(T(0.5) * P.gamma / betah);
+    const T a = m_sqrt(qdiv<FAST>(T(0.5) * P.gamma, betah));
     const T vsq = vn * vn + vt * vt;
-    const T H = a * a / P.gm1 + T(0.5) * vsq;
-    const T c1 = m_sqrt(rhoh / P.gamma);
+    const T H = qdiv<FAST>(a * a, P.gm1) + T(0.5) * vsq;
+    const T c1 = m_sqrt(qdiv<FAST>(rhoh, P.gamma));
     const T psq = m_sqrt(p);
     const T S0 = P.s05 * c1, S1 = P.sgm1 * c1, S2 = psq;
     // scaled eigenvectors (rows rho, n, t, E; waves 0, 1, 2, 4)
@@ -448,7 +456,8 @@ struct Euler2Ph {
       L0 = m_abs(vn - a);
       L4 = m_abs(vn + a);
       if (FV == FV_GENERIC && P.ec1_fix && fl != 106) {
-        const T al = m_sqrt(T(0.5) * P.gamma / bk[L]), ar = m_sqrt(T(0.5) * P.gamma / bk[R]);
+        const T al = m_sqrt(qdiv<FAST>(T(0.5) * P.gamma, bk[L])),
+                ar = m_sqrt(qdiv<FAST>(T(0.5) * P.gamma, bk[R]));
         L0 = L0 + T(1.0 / 6.0) * m_abs((un[L] - al) - (un[R] - ar));
         L4 = L4 + T(1.0 / 6.0) * m_abs(un[L] - un[R]);
       }

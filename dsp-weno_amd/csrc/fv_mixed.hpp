@@ -481,14 +481,21 @@ __device__ __forceinline__ void jumps_fp32(const Phys<double>& P, const float* d
 // u < 1e-2 is made as (b-a)^2 < 1e-2 (b+a)^2, so each branch needs one
 // reciprocal of its own denominator; both are evaluated and one is selected
 // (no divergent branch in a wave).
-__device__ __forceinline__ double logavg_ln(double a, double b, double la, double lb) {
+// The second reciprocal serves the mean's inverse as well, when asked for:
+// 1/mean is 2F/(a+b) = 2 F r in the series branch and (ln b - ln a)/(b - a)
+// in the other, whose reciprocal of (b - a) is the one taken there.
+__device__ __forceinline__ double logavg_ln(double a, double b, double la, double lb,
+                                            double* inv = nullptr) {
   const double dab = b - a, sab = b + a;
   const bool series = dab * dab < 1.0e-2 * (sab * sab);
   const double r = rcp64(series ? sab : lb - la);
   const double f = dab * r;
   const double u = f * f;
   const double F = 1.0 + u * (1.0 / 3.0) + (u * u) * (1.0 / 5.0) + (u * u * u) * (1.0 / 7.0);
-  return series ? 0.5 * sab * rcp64(F) : dab * r;
+  if (!inv) return series ? 0.5 * sab * rcp64(F) : dab * r;
+  const double q = rcp64(series ? F : dab);
+  *inv = series ? 2.0 * F * r : (lb - la) * q;
+  return series ? 0.5 * sab * q : dab * r;
 }
 
 // ---------------------------------------------------------------- the face
@@ -570,8 +577,9 @@ struct Euler2Mixed {
   __device__ static void kepec(const Phys<float>& P, double rl, double nl, double tl, double bl,
                                double lrl, double lbl, double rr, double nr, double tr, double br,
                                double lrr, double lbr, double F[4], double* aux = nullptr) {
+    double ibeta;
     const double rho = logavg_ln(rl, rr, lrl, lrr);
-    const double beta = logavg_ln(bl, br, lbl, lbr);
+    const double beta = logavg_ln(bl, br, lbl, lbr, &ibeta);
     const double vel2 = 0.5 * ((nl * nl + tl * tl) + (nr * nr + tr * tr));
     const double p = 0.5 * (rl + rr) * rcp64(bl + br);
     const double vn = 0.5 * (nl + nr), vt = 0.5 * (tl + tr);
@@ -579,13 +587,14 @@ struct Euler2Mixed {
     F[0] = mass;
     F[1] = p + vn * mass;
     F[2] = vt * mass;
-    F[3] = 0.5 * (rcp64(P.gm1_d * beta) - vel2) * mass + (F[1] * vn + F[2] * vt);
+    F[3] = 0.5 * (ibeta * P.inv_gm1_d - vel2) * mass + (F[1] * vn + F[2] * vt);
     if (aux) {
       aux[0] = rho;
       aux[1] = beta;
       aux[2] = p;
       aux[3] = vn;
       aux[4] = vt;
+      aux[5] = ibeta;
     }
   }
 
@@ -638,7 +647,7 @@ struct Euler2Mixed {
     // copies of the eigenvectors live, then the FP64 EC4 flux from the
     // records re-read out of LDS.
     // ---- central two-point flux F(1,2) and its means (kepec.cc:10-48)
-    double Fc[4], aux[5];
+    double Fc[4], aux[6];
     {
       const int c1 = base + stride, c2 = base + 2 * stride;
       kepec(P, ld[R_RHO * NB + c1], ld[rn * NB + c1], ld[rt * NB + c1], ld[R_BETA * NB + c1],
@@ -654,12 +663,12 @@ struct Euler2Mixed {
     float dZ[4][3], Z1[4];
     float vnf, af, s0, s1, Rn0f, Rt0f, RE0f, Rn1f, Rt1f, RE1f, Rt2f, RE2f, Rn4f, RE4f;
     {
-      const double rhoh = aux[0], betah = aux[1], ph = aux[2], vn = aux[3], vt = aux[4];
+      const double rhoh = aux[0], ph = aux[2], vn = aux[3], vt = aux[4], ibetah = aux[5];
       // c1 and S2 in FP32 (one instruction each instead of a scaled FP64
       // sequence): they scale whole rows of R~^T, so their rounding scales a
       // projected jump without disturbing the cancellation inside it.  The
       // sound speed a sits inside the acoustic rows and stays FP64.
-      const double a = sqrt(0.5 * P.gamma_d * rcp64(betah));
+      const double a = sqrt(0.5 * P.gamma_d * ibetah);
       const double vsq = vn * vn + vt * vt;
       const double H = a * a * P.inv_gm1_d + 0.5 * vsq;
       const double c1 = (double)fast::sqrt((float)(rhoh * P.inv_gamma_d));

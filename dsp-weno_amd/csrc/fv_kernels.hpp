@@ -306,8 +306,7 @@ struct Tile {
   }
 };
 
-#ifndef TECNO_HOST_BUILD
-// Tile of this workgroup.  The dispatcher deals workgroups out round-robin
+// Tile of a workgroup.  The dispatcher deals workgroups out round-robin
 // over the chip's 8 XCDs (workgroup i to XCD i mod 8), each with its own L2,
 // so neighbouring tiles of the launch grid land on different XCDs and the
 // halo cells that two tiles share are fetched from HBM by both.  The linear
@@ -318,26 +317,25 @@ struct Tile {
 // every XCD still gets rows from all over the grid (the cost of a tile
 // depends on the flow in it, so contiguous eighths of the grid would leave
 // XCDs idle).  Tiles past the last whole round of chunks are split into 8
-// contiguous runs.  A bijection of [0, tiles) for any grid.
+// contiguous runs.  A bijection of [0, gx gy gz) for any grid (host test).
 #ifndef TECNO_XCD_ROWS
 #define TECNO_XCD_ROWS 2
 #endif
+__device__ __forceinline__ unsigned xcd_tile(unsigned lin, unsigned gx, unsigned gy, unsigned gz) {
+  if (TECNO_XCD_ROWS <= 0) return lin;
+  const unsigned T = gx * gy * gz;
+  const unsigned C = (unsigned)TECNO_XCD_ROWS * (gz > 1 ? gx * gy : gx);
+  const unsigned xcd = lin % 8u, k = lin / 8u, q = T / 8u, r = T % 8u;
+  const unsigned J = q / C;  // whole chunks that every XCD takes
+  if (k < J * C) return ((k / C) * 8u + xcd) * C + k % C;
+  const unsigned k2 = k - J * C, q2 = q - J * C;
+  return 8u * J * C + (xcd < r ? xcd * (q2 + 1u) : r * (q2 + 1u) + (xcd - r) * q2) + k2;
+}
+
+#ifndef TECNO_HOST_BUILD
 __device__ __forceinline__ void tile_of_block(int& bx, int& by, int& bz) {
   const unsigned gx = gridDim.x, gy = gridDim.y;
-  const unsigned T = gx * gy * gridDim.z;
-  const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-  unsigned t = lin;
-  if (TECNO_XCD_ROWS > 0) {
-    const unsigned C = (unsigned)TECNO_XCD_ROWS * (gridDim.z > 1 ? gx * gy : gx);
-    const unsigned xcd = lin % 8u, k = lin / 8u, q = T / 8u, r = T % 8u;
-    const unsigned J = q / C;  // whole chunks that every XCD takes
-    if (k < J * C) {
-      t = ((k / C) * 8u + xcd) * C + k % C;
-    } else {
-      const unsigned k2 = k - J * C, q2 = q - J * C;
-      t = 8u * J * C + (xcd < r ? xcd * (q2 + 1u) : r * (q2 + 1u) + (xcd - r) * q2) + k2;
-    }
-  }
+  const unsigned t = xcd_tile(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), gx, gy, gridDim.z);
   bx = (int)(t % gx);
   by = (int)((t / gx) % gy);
   bz = (int)(t / (gx * gy));

@@ -345,8 +345,9 @@ struct Euler3Ph {
                               T F[5], unsigned& nfb) {
     constexpr int S = 2 * G;
     const int L = G - 1, R = G;
-    T n[3] = {T(0), T(0), T(0)};
-    n[ax] = T(1);
+    // unit normal of axis ax, without indexing by the lane's axis (a register
+    // array indexed that way would live in scratch memory)
+    const T n[3] = {ax == 0 ? T(1) : T(0), ax == 1 ? T(1) : T(0), ax == 2 ? T(1) : T(0)};
     const T area = P.area[ax];
     Pr3<T> q[S];
 #pragma unroll

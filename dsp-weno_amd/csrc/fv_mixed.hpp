@@ -394,13 +394,6 @@ __device__ __forceinline__ float sel4(int i, float a, float b, float c, float d)
 // v_mfma_f32_32x32x16 per layer
 enum { M3_NONE = 0, M3_SPW = 1, M3_DSP_SIMT = 2, M3_DSP_BF16 = 3, M3_DSP_FP16 = 4 };
 
-template <int NC>
-__device__ __forceinline__ float selc(int i, const float* a) {
-  float r = a[NC - 1];
-#pragma unroll
-  for (int k = NC - 2; k >= 0; --k) r = (i == k) ? a[k] : r;
-  return r;
-}
 
 // Reconstructed jumps <<Z_i>> of NC characteristic components from their
 // three FP32 cell jumps d0, d1, d2 and centre values z1: SP-WENO's jump
@@ -420,54 +413,81 @@ __device__ __forceinline__ void jumps_fp32(const Phys<double>& P, const float* d
     for (int i = 0; i < NC; ++i)
       jmp[i] = fast::spw_jump(d0[i], d1[i], d2[i], z1[i], z1[i] + d1[i], P.corr);
   } else {
-    unsigned bad = 0;
 #ifndef TECNO_HOST_BUILD
     [[maybe_unused]] mfma::Net5<MIX == M3_DSP_FP16> net;
     if constexpr (MIX != M3_DSP_SIMT) net.load(P.mfma_w[MIX == M3_DSP_FP16 ? 1 : 0], threadIdx.x & 63);
 #endif
+    // The loop stays rolled (the network's registers); the components move
+    // through slot 0 of rotating copies of the jumps, and each result enters
+    // the last slot of jr, which the remaining rotations carry to its place.
+    // (Indexing the arrays by the loop counter instead puts them in scratch
+    // memory.)
+    float a0[NC], a1[NC], a2[NC], az[NC], jr[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      a0[k] = d0[k];
+      a1[k] = d1[k];
+      a2[k] = d2[k];
+      az[k] = z1[k];
+      jr[k] = 0.0f;
+    }
 #pragma unroll 1
     for (int i = 0; i < NC; ++i) {
-      const float e1z = selc<NC>(i, d1);
+      const float e1z = a1[0];
       const bool skip = !(fabsf(e1z) >= 1e-15f);
+      float j = 0.0f;
 #ifndef TECNO_HOST_BUILD
-      if (__all(skip)) continue;  // zero centre jump on every lane: jump 0
-#endif
-      // a skipped lane is evaluated with a centre jump of 1: its ratios stay
-      // finite, so the partner lane of its matrix column sees finite
-      // products; its jump is set to 0 below
-      const float e1 = skip ? 1.0f : e1z;
-      const float e0 = selc<NC>(i, d0), e2 = selc<NC>(i, d2), y1 = selc<NC>(i, z1);
-      const float y2 = y1 + e1, y0 = y1 - e0, y3 = y2 + e2;
-      const float maxabs = fmaxf(fmaxf(fabsf(y0), fabsf(y1)), fmaxf(fabsf(y2), fabsf(y3)));
-      float st[5], tm, tp, maxd;
-      fast::features_nv(e0, e1, e2, maxabs, st, tm, tp, maxd);
-      float j;
-#ifndef TECNO_HOST_BUILD
-      if constexpr (MIX != M3_DSP_SIMT) {
-        float lg[5];
-        net.eval(st, lg);
-        j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
-      } else
+      if (!__all(skip))  // else a zero centre jump on every lane: jump 0
 #endif
       {
-        fast::body5(P.net32, st);
-        j = fast::sign_round(fast::head_combine(st, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
+        // a skipped lane is evaluated with a centre jump of 1: its ratios stay
+        // finite, so the partner lane of its matrix column sees finite
+        // products; its jump is set to 0 below
+        const float e1 = skip ? 1.0f : e1z;
+        const float e0 = a0[0], e2 = a2[0], y1 = az[0];
+        const float y2 = y1 + e1, y0 = y1 - e0, y3 = y2 + e2;
+        const float maxabs = fmaxf(fmaxf(fabsf(y0), fabsf(y1)), fmaxf(fabsf(y2), fabsf(y3)));
+        float st[5], tm, tp, maxd;
+        fast::features_nv(e0, e1, e2, maxabs, st, tm, tp, maxd);
+#ifndef TECNO_HOST_BUILD
+        if constexpr (MIX != M3_DSP_SIMT) {
+          float lg[5];
+          net.eval(st, lg);
+          j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
+        } else
+#endif
+        {
+          fast::body5(P.net32, st);
+          j = fast::sign_round(fast::head_combine(st, tm, tp, maxd, e0, e1, e2), e0, e1, e2);
+        }
+        if (skip) j = 0.0f;
+        // sign property: SP-WENO replaces a network jump whose sign disagrees
+        // with the cell jump, here, so that no copy of the stencil jumps has
+        // to outlive the loop (a wave branches into it only when one of its
+        // lanes needs it)
+        const bool fail = P.sign_fallback && j * e1 < 0.0f;
+#ifndef TECNO_HOST_BUILD
+        if (__any(fail))
+#endif
+        {
+          if (fail) {
+            j = fast::spw_jump(e0, e1, e2, y1, y1 + e1, 0);
+            ++nfb;
+          }
+        }
       }
-      if (skip) j = 0.0f;
-      bad |= (j * e1 < 0.0f) ? (1u << i) : 0u;
 #pragma unroll
-      for (int k = 0; k < NC; ++k) jmp[k] = (k == i) ? j : jmp[k];
+      for (int k = 0; k + 1 < NC; ++k) {
+        a0[k] = a0[k + 1];
+        a1[k] = a1[k + 1];
+        a2[k] = a2[k + 1];
+        az[k] = az[k + 1];
+        jr[k] = jr[k + 1];
+      }
+      jr[NC - 1] = j;
     }
-    if (!P.sign_fallback) bad = 0;
-    while (bad) {
-      const int i = __builtin_ctz(bad);
-      bad &= bad - 1;
-      const float e1 = selc<NC>(i, d1), y1 = selc<NC>(i, z1);
-      const float j = fast::spw_jump(selc<NC>(i, d0), e1, selc<NC>(i, d2), y1, y1 + e1, 0);
 #pragma unroll
-      for (int k = 0; k < NC; ++k) jmp[k] = (k == i) ? j : jmp[k];
-      ++nfb;
-    }
+    for (int k = 0; k < NC; ++k) jmp[k] = jr[k];
   }
 }
 

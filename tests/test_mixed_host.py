@@ -62,7 +62,10 @@ int main() {
   FILE* f = std::fopen(std::getenv("LOGAVG_OUT"), "w");
   for (int it = 0; it < 20000; ++it) {
     double a = V(h), b = (it % 3 == 0) ? a * (1.0 + (V(h) - 2.6) * 1e-3) : V(h);
-    std::fprintf(f, "%.17g %.17g %.17g\n", a, b, logavg_ln(a, b, std::log(a), std::log(b)));
+    double ib;
+    const double m2 = logavg_ln(a, b, std::log(a), std::log(b), &ib);
+    std::fprintf(f, "%.17g %.17g %.17g %.17g %.17g %.17g\n", a, b, logavg_ln(a, b, std::log(a), std::log(b)),
+                 logavg_fast(a, b), m2, ib);
   }
   std::fclose(f);
   // record logarithm: largest error in ulp of std::log over 24 decades, near 1,
@@ -123,6 +126,17 @@ def test_logavg_from_cell_logs_matches_oracle(harness_output):
     ref = orc.logavg(vals[:, 0], vals[:, 1])
     rel = np.abs(vals[:, 2] - ref) / np.abs(ref)
     assert rel.max() <= 4e-15
+
+
+def test_logavg_fast_and_inverse_match_oracle(harness_output):
+    """logavg_fast (the FP32 paths' FP64 flux: one reciprocal for f, log64 in
+    the logarithm branch) and logavg_ln's mean-and-inverse form against the
+    oracle's logavg (misc_func.h:14-32), on pairs that take both branches."""
+    _, vals = harness_output
+    ref = orc.logavg(vals[:, 0], vals[:, 1])
+    assert (np.abs(vals[:, 3] - ref) / ref).max() <= 8e-15
+    assert (np.abs(vals[:, 4] - ref) / ref).max() <= 8e-15
+    assert (np.abs(vals[:, 5] * ref - 1.0)).max() <= 8e-15
 
 
 def test_record_log_within_few_ulp(harness_output):

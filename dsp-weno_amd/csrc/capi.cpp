@@ -1364,6 +1364,7 @@ struct EnsGroupH {
   int prec = ENS_FP64;  // ENS_* of the members (their cfg.precision)
   int dims = 1;
   int variant = 0;
+  int spw = 1;  // every member reconstructs with the SP-WENO family
   std::vector<int> members;
   void* dprobs = nullptr;
   EnsWork* dwork = nullptr;
@@ -1412,7 +1413,7 @@ static int ens_build_work(tecno_ens* e, EnsGroupH& G) {
   G.nwork = (int)w.size();
   HIPCHK(hipMalloc(&G.dwork, w.size() * sizeof(EnsWork)));
   HIPCHK(hipMemcpy(G.dwork, w.data(), w.size() * sizeof(EnsWork), hipMemcpyHostToDevice));
-  const int resident = ens_grid(G.prec, G.dims, G.variant, e->n_cu);
+  const int resident = ens_grid(G.prec, G.dims, G.variant, G.spw, e->n_cu);
   G.grid = std::max(1, std::min(G.nwork, resident * G.gridf));
   ens_drop_graph(e);
   return 0;
@@ -1458,7 +1459,8 @@ static hipError_t ens_step(tecno_ens* e, int only_group) {
     for (int gi = 0; gi < (int)e->g.size() && r == hipSuccess; ++gi) {
       if (only_group >= 0 && gi != only_group) continue;
       const EnsGroupH& G = e->g[gi];
-      r = ens_stage(e->s, G.prec, G.dims, G.variant, G.dprobs, G.dwork, G.nwork, st, G.grid);
+      r = ens_stage(e->s, G.prec, G.dims, G.variant, G.spw, G.dprobs, G.dwork, G.nwork, st,
+                    G.grid);
     }
   if (r == hipSuccess) r = ens_control(e->s, e->dctl, n, 1);
   return r;
@@ -1520,6 +1522,7 @@ int tecno_ens_create(const tecno_cfg* cfgs, int n, tecno_ens** out) {
       e->g[gi].dims = dims;
     }
     e->g[gi].members.push_back(i);
+    if (c->P64.recon != REC_SPWENO) e->g[gi].spw = 0;
   }
   int dev = 0;
   hipDeviceProp_t prop;

@@ -7,5 +7,8 @@ namespace tecno {
 template hipError_t ens_launch<ScalarPh<float, 2>>(hipStream_t, int, int, const void*,
                                                     const EnsWork*, int, int, int);
 template int ens_grid_of<ScalarPh<float, 2>>(int, int, int);
+template hipError_t ens_launch<ScalarPh<float, 2, 1>>(hipStream_t, int, int, const void*,
+                                                       const EnsWork*, int, int, int);
+template int ens_grid_of<ScalarPh<float, 2, 1>>(int, int, int);
 
 }  // namespace tecno

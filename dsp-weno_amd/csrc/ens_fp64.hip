@@ -7,6 +7,9 @@ namespace tecno {
 template hipError_t ens_launch<ScalarPh<double, 2>>(hipStream_t, int, int, const void*,
                                                      const EnsWork*, int, int, int);
 template int ens_grid_of<ScalarPh<double, 2>>(int, int, int);
+template hipError_t ens_launch<ScalarPh<double, 2, 1>>(hipStream_t, int, int, const void*,
+                                                        const EnsWork*, int, int, int);
+template int ens_grid_of<ScalarPh<double, 2, 1>>(int, int, int);
 template hipError_t ens_launch<ScalarMixedPh<2>>(hipStream_t, int, int, const void*,
                                                  const EnsWork*, int, int, int);
 template int ens_grid_of<ScalarMixedPh<2>>(int, int, int);
@@ -14,6 +17,9 @@ template int ens_grid_of<ScalarMixedPh<2>>(int, int, int);
 extern template hipError_t ens_launch<ScalarPh<float, 2>>(hipStream_t, int, int, const void*,
                                                           const EnsWork*, int, int, int);
 extern template int ens_grid_of<ScalarPh<float, 2>>(int, int, int);
+extern template hipError_t ens_launch<ScalarPh<float, 2, 1>>(hipStream_t, int, int, const void*,
+                                                             const EnsWork*, int, int, int);
+extern template int ens_grid_of<ScalarPh<float, 2, 1>>(int, int, int);
 
 __global__ void k_ens_control(const EnsCtl* cs, int n, int finish) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -66,24 +72,30 @@ hipError_t ens_count_active(hipStream_t s, const EnsCtl* c, int n, int* out) {
   return hipGetLastError();
 }
 
-hipError_t ens_stage(hipStream_t s, int prec, int dims, int variant, const void* probs,
+hipError_t ens_stage(hipStream_t s, int prec, int dims, int variant, int spw, const void* probs,
                      const EnsWork* work, int nwork, int stage, int grid) {
   if (nwork <= 0) return hipSuccess;
   switch (prec) {
     case ENS_FP32:
-      return ens_launch<ScalarPh<float, 2>>(s, dims, variant, probs, work, nwork, stage, grid);
+      return spw ? ens_launch<ScalarPh<float, 2, 1>>(s, dims, variant, probs, work, nwork, stage, grid)
+                 : ens_launch<ScalarPh<float, 2>>(s, dims, variant, probs, work, nwork, stage, grid);
     case ENS_FP32_STATE64:
       return ens_launch<ScalarMixedPh<2>>(s, dims, variant, probs, work, nwork, stage, grid);
     default:
-      return ens_launch<ScalarPh<double, 2>>(s, dims, variant, probs, work, nwork, stage, grid);
+      return spw ? ens_launch<ScalarPh<double, 2, 1>>(s, dims, variant, probs, work, nwork, stage, grid)
+                 : ens_launch<ScalarPh<double, 2>>(s, dims, variant, probs, work, nwork, stage, grid);
   }
 }
 
-int ens_grid(int prec, int dims, int variant, int n_cu) {
+int ens_grid(int prec, int dims, int variant, int spw, int n_cu) {
   switch (prec) {
-    case ENS_FP32: return ens_grid_of<ScalarPh<float, 2>>(dims, variant, n_cu);
+    case ENS_FP32:
+      return spw ? ens_grid_of<ScalarPh<float, 2, 1>>(dims, variant, n_cu)
+                 : ens_grid_of<ScalarPh<float, 2>>(dims, variant, n_cu);
     case ENS_FP32_STATE64: return ens_grid_of<ScalarMixedPh<2>>(dims, variant, n_cu);
-    default: return ens_grid_of<ScalarPh<double, 2>>(dims, variant, n_cu);
+    default:
+      return spw ? ens_grid_of<ScalarPh<double, 2, 1>>(dims, variant, n_cu)
+                 : ens_grid_of<ScalarPh<double, 2>>(dims, variant, n_cu);
   }
 }
 

@@ -47,12 +47,14 @@ __host__ __device__ constexpr int ens_ty(int dims, int v) {
 // dt of every active problem (k_dt per problem) / end of step (k_finish)
 hipError_t ens_control(hipStream_t s, const EnsCtl* c, int n, int finish);
 // one SSPRK3 stage (0, 1, 2) of every problem of a group of precision
-// kind prec (ENS_FP64, ENS_FP32, ENS_FP32_STATE64)
+// kind prec (ENS_FP64, ENS_FP32, ENS_FP32_STATE64); spw: every member
+// reconstructs with the SP-WENO family (the kernel with that
+// reconstruction only)
 enum { ENS_FP64 = 0, ENS_FP32 = 1, ENS_FP32_STATE64 = 2 };
-hipError_t ens_stage(hipStream_t s, int prec, int dims, int variant, const void* probs,
+hipError_t ens_stage(hipStream_t s, int prec, int dims, int variant, int spw, const void* probs,
                      const EnsWork* work, int nwork, int stage, int grid);
 // persistent grid size for a group's kernel: resident blocks on the device
-int ens_grid(int prec, int dims, int variant, int n_cu);
+int ens_grid(int prec, int dims, int variant, int spw, int n_cu);
 // number of problems still advancing (written to *out on the device)
 hipError_t ens_count_active(hipStream_t s, const EnsCtl* c, int n, int* out);
 

@@ -8,7 +8,12 @@ namespace tecno {
 
 // ================================================================ physics
 // -------- scalar (enflo_scalar): record = u
-template <typename T, int G>
+// RF 1: the face reconstructs with the SP-WENO family only (sp_weno,
+// sp_wenoc, sp_weno_corr2: P.corr), for problems that use it.  Without the
+// other reconstructions compiled in -- the FP64 network above all -- the FP64
+// 2D kernels need 88-92 VGPRs instead of 161-166 and run five waves per SIMD
+// instead of three, which their chains of FP64 divisions need.
+template <typename T, int G, int RF = 0>
 struct ScalarPh {
   using TS = T;  // state in HBM
   static constexpr int MIN_WAVES = 1;
@@ -74,7 +79,10 @@ struct ScalarPh {
     if (tecno(P)) {
       const T D = diss_coef(P, s, ax);
       T uij, uji;
-      recon_any(P, s, G, uij, uji);
+      if constexpr (RF == 1)
+        sp_weno(s[G - 2], s[G - 1], s[G], s[G + 1], P.corr, uij, uji);
+      else
+        recon_any(P, s, G, uij, uji);
       f -= T(0.5) * D * (uji - uij);
     }
     F[0] = f * P.area[ax];

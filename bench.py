@@ -403,11 +403,13 @@ def vortex_state(n, t=0.0, strength=5.0, gamma=1.4):
 def l1_gate(tecno, ns=(128, 256, 512), t_end=1.0, net_file=None):
     """Equal-error gate on the isentropic vortex (exact solution): L1(rho) at
     t = 1 on each mesh and the observed orders, for the headline path (mixed,
-    DSP-WENO, bf16 network), the same method in FP64 (reference arithmetic)
-    and the FP64 SP-WENO path."""
+    DSP-WENO, bf16 network), the same method in FP64 (reference arithmetic),
+    the FP64 SP-WENO path and SP-WENO with the headline's arithmetic split
+    (FP32 jump-form reconstruction on the FP64 state)."""
     paths = {"headline": (tecno.FP32, "sp_weno_dl", tecno.MLP_BF16),
              "fp64_dsp": (tecno.FP64, "sp_weno_dl", tecno.MLP_FP64),
-             "fp64_spweno": (tecno.FP64, "sp_weno", tecno.MLP_FP64)}
+             "fp64_spweno": (tecno.FP64, "sp_weno", tecno.MLP_FP64),
+             "mixed_spweno": (tecno.FP32, "sp_weno", tecno.MLP_FP64)}
     out = {"case": "isentropic vortex, strength 5, periodic [-5,5]^2, t=1, L1(rho)", "n": list(ns)}
     for name, (prec, rec, mlp) in paths.items():
         errs = []
@@ -437,6 +439,9 @@ def l1_gate(tecno, ns=(128, 256, 512), t_end=1.0, net_file=None):
                     all(abs(x - y) <= 0.05 for x, y in zip(oa, orf)))
     out["pass_vs_fp64_spweno"] = same("headline", "fp64_spweno")
     out["pass_vs_fp64_dsp"] = same("headline", "fp64_dsp")
+    out["mixed_spweno_l1_ratio_vs_fp64_spweno"] = [
+        round(x / y, 4) for x, y in zip(out["mixed_spweno"]["l1"], b)]
+    out["mixed_spweno_pass_vs_fp64_spweno"] = same("mixed_spweno", "fp64_spweno")
     out["rule"] = "L1 within 2% on every mesh and convergence orders within 0.05"
     return out
 
@@ -630,7 +635,9 @@ def main():
              "mixed_fp16_mfma": (tecno.FP32, "sp_weno_dl", tecno.MLP_FP16, "shipped"),
              "mixed_fp32_simt": (tecno.FP32, "sp_weno_dl", tecno.MLP_FP32, "shipped"),
              "mixed_bf16_mfma_xavier": (tecno.FP32, "sp_weno_dl", tecno.MLP_BF16, "xavier"),
-             "fp64_dsp_xavier": (tecno.FP64, "sp_weno_dl", tecno.MLP_FP64, "xavier")}
+             "fp64_dsp_xavier": (tecno.FP64, "sp_weno_dl", tecno.MLP_FP64, "xavier"),
+             # SP-WENO with the headline's arithmetic split, no network
+             "mixed_spweno": (tecno.FP32, "sp_weno", tecno.MLP_FP64, "shipped")}
     sel = {"": tuple(paths), "headline": ("headline",), "fp64": ("fp64_spweno",),
            "fp64dsp": ("fp64_dsp",), "matrix": tuple(paths)}[args.only]
     for name in sel:
@@ -640,7 +647,8 @@ def main():
         if args.warmup:
             s.run_steps(args.warmup, final_time=1.0, t_stamps=1)
         res[name] = time_steps(s, n, args.steps)
-        if rec == "sp_weno_dl" and args.only in ("", "matrix"):
+        if args.only in ("", "matrix") and (rec == "sp_weno_dl" or
+                                            name in ("fp64_spweno", "mixed_spweno")):
             finals[name] = s.get_state()
         s.close()
         print(f"# {name} from the spun-up state (t={t_spun:.4f}): {res[name]}", file=sys.stderr,
@@ -732,6 +740,14 @@ def main():
                                                        args.warmup + args.steps)
         matrix[name] = row
     out["network_matrix"] = matrix
+    # SP-WENO with the headline's arithmetic split against FP64 SP-WENO: speed
+    # and deviation on the KH state; its equal-error flag comes from the gate
+    m = res["mixed_spweno"]
+    out["mixed_spweno"] = {
+        "stage_ms": m["stage_ms"], "cell_updates_per_s": m["cell_updates_per_s"],
+        "speedup_vs_fp64_spweno": m["cell_updates_per_s"] / b["cell_updates_per_s"],
+        "deviation_vs_fp64_spweno": state_deviation(finals["mixed_spweno"], finals["fp64_spweno"],
+                                                    args.warmup + args.steps)}
     if args.only == "matrix":
         print(json.dumps(out["network_matrix"]), flush=True)
         return
@@ -741,6 +757,7 @@ def main():
     # mesh of the vortex and the same convergence order (to 0.05)
     out["equal_l1_vs_fp64_spweno"] = gate["pass_vs_fp64_spweno"]
     out["equal_l1_vs_fp64_dsp"] = gate["pass_vs_fp64_dsp"]
+    out["mixed_spweno"]["equal_l1_vs_fp64_spweno"] = gate["mixed_spweno_pass_vs_fp64_spweno"]
     print(f"# equal-error gate: {gate}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()

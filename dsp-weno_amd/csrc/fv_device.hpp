@@ -170,7 +170,10 @@ __device__ __forceinline__ void spw_cp(T t0, T t1, T del1, T scale, bool want_co
       corr += T(-0.25) * m_cube(m_min(m_abs(del1 / scale), m_abs(del1))) / (T(1) - t0);
   } else if (m_abs(t0 - T(1)) > eps && m_abs(psi + T(1)) < eps) {
     C = T(0);
-  } else if (m_abs(t0 - T(1)) < eps || (psi > -eps && m_abs(t0) < T(1) + eps)) {
+  } else if (m_abs(t0 - T(1)) < eps ||
+             (psi > -eps && (sizeof(T) == 4 ? m_abs(t0) <= T(1) : m_abs(t0) < T(1) + eps))) {
+    // |t0| < 1 + eps; in FP32 1 + 1e-13 rounds to 1, so the floats below
+    // 1 + 1e-13 are those <= 1 (|t0| = 1 exactly: equal or alternating jumps)
     C = T(-3.0 / 8.0);
   } else if (psi > -eps && m_abs(t0) > T(1)) {
     C = T(1.0 / 8.0);

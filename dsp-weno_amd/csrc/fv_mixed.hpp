@@ -70,12 +70,6 @@ __device__ __forceinline__ float logavg(float a, float b) {
 }
 
 // ---------------------------------------------------------------- SP-WENO
-__device__ __forceinline__ float spw_fp(float t0, float t1) {
-  const float eps = 1.0e-13f;
-  float psi = (fabsf(t0 - 1.0f) < eps) ? 0.0f : quot(1.0f - t1, 1.0f - t0);
-  return (fabsf(t0 - 1.0f) > eps && fabsf(psi + 1.0f) > eps) ? rcp(1.0f + psi) : 1.0f;
-}
-
 __device__ __forceinline__ float spw_cp(float t0, float t1, float del1, float scale, bool wc,
                                         float& corr) {
   const float eps = 1.0e-13f;
@@ -87,15 +81,21 @@ __device__ __forceinline__ float spw_cp(float t0, float t1, float del1, float sc
   float C = 0.0f;
   corr = 0.0f;
   if (fabsf(psi + 1.0f) > eps && fabsf(t0 - 1.0f) > eps && psi < 0.0f) {
-    const float a = spw_fp(t0, t1), b = spw_fp(t1, t0);
-    C = a * rcp(a * a + b * b) * 0.125f;
+    // Cp's a / (a^2 + b^2) / 8 (limiter.h:152-190) with a = fp(t0, t1) =
+    // 1 / (1 + psi) and b = fp(t1, t0) = 1 / (1 + 1/psi) (limiter.h:103) is
+    // (1 + psi) / (1 + psi^2) / 8: one reciprocal instead of two quotients
+    // and three reciprocals
+    C = 0.125f * (1.0f + psi) * rcp(fmaf(psi, psi, 1.0f));
     if (wc) {
       const float m = fminf(fabsf(del1 * rcp(scale)), fabsf(del1));
       corr = -0.25f * m * m * m * rcp(1.0f - t0);
     }
   } else if (fabsf(t0 - 1.0f) > eps && fabsf(psi + 1.0f) < eps) {
     C = 0.0f;
-  } else if (fabsf(t0 - 1.0f) < eps || (psi > -eps && fabsf(t0) < 1.0f + eps)) {
+  } else if (fabsf(t0 - 1.0f) < eps || (psi > -eps && fabsf(t0) <= 1.0f)) {
+    // |t0| < 1 + eps of the FP64 test: 1 + 1e-13 rounds to 1 in FP32, and
+    // the floats below 1 + 1e-13 are those <= 1 (|t0| = 1 exactly: equal or
+    // alternating jumps)
     C = -0.375f;
   } else if (psi > -eps && fabsf(t0) > 1.0f) {
     C = 0.125f;

@@ -51,8 +51,9 @@ XAVIER_SEED = 6008  # the trainer's default seed (config.py)
 
 
 def make_solver(tecno, n, precision, recon, mlp_dtype, weights="shipped"):
-    """weights: "shipped" (the net_file) or "xavier" (seeded Xavier-uniform
-    weights of the same 5-5-5-5-5 architecture, tecno_init_net)"""
+    """weights: "shipped" (the net_file) or "xavier" (the trainer's seeded
+    initial network of the same 5-5-5-5-5 architecture: Xavier-uniform
+    weights as PyTorch draws them under manual_seed(6008), tecno_init_net)"""
     cfg = tecno.make_cfg(n=(n, n, 2), dims=2, lo=(0, 0, 0), hi=(1, 1, 1), recon=recon,
                          flux="kepes_tecno4_roe", bc="periodic", cfl=0.5, precision=precision,
                          mlp_dtype=mlp_dtype, sign_fallback=1)

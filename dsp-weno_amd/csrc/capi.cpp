@@ -17,6 +17,7 @@
 #include "diag.hpp"
 #include "ensemble.hpp"
 #include "mlp.hpp"
+#include "netinit.hpp"
 
 namespace tecno {
 extern template struct Launch<double>;
@@ -1342,17 +1343,11 @@ int tecno_init_net(tecno_ctx* c, int nl, const int32_t* widths, uint64_t seed) {
   if (nl < 1 || nl > TECNO_NET_MAXL) return fail(TECNO_EINVAL, "bad layer count");
   if (widths[0] != 5 || widths[nl] != 5)
     return fail(TECNO_EINVAL, "the network maps 5 inputs to 5 vertex weights");
-  std::mt19937_64 rng(seed);
-  std::vector<double> W, b;
-  for (int l = 0; l < nl; ++l) {
-    const int ni = widths[l], no = widths[l + 1];
-    if (ni < 1 || no < 1 || ni > TECNO_NET_MAXW || no > TECNO_NET_MAXW)
+  for (int l = 0; l <= nl; ++l)
+    if (widths[l] < 1 || widths[l] > TECNO_NET_MAXW)
       return fail(TECNO_EINVAL, "layer widths must be 1..16");
-    const double aw = std::sqrt(6.0 / (double)(ni + no)), ab = 1.0 / std::sqrt((double)ni);
-    std::uniform_real_distribution<double> dw(-aw, aw), db(-ab, ab);
-    for (int q = 0; q < ni * no; ++q) W.push_back(dw(rng));
-    for (int q = 0; q < no; ++q) b.push_back(db(rng));
-  }
+  std::vector<double> W, b;
+  init_net_torch(nl, widths, seed, W, b);
   return tecno_set_net(c, nl, widths, W.data(), b.data());
 }
 

@@ -184,11 +184,13 @@ int tecno_get_net(const tecno_ctx* ctx, int* n_layers, int32_t* widths,
  * "Layer k" / "BEGIN" / one out-row per line / "END", then "Biases" likewise,
  * values as %.18e). */
 int tecno_save_net(const tecno_ctx* ctx, const char* net_file);
-/* Seeded initial network of n_layers with the given widths (5 in, 5 out):
- * weights Xavier-uniform U(+-sqrt(6/(fan_in+fan_out))) as the trainer's
- * nn.init.xavier_uniform_ (DSP-WENO_network_trainer.py:58-60), biases
- * U(+-1/sqrt(fan_in)) as nn.Linear's default.  The random stream is this
- * library's own (a seed does not reproduce PyTorch's draws). */
+/* Seeded initial network of n_layers with the given widths (5 in, 5 out),
+ * the trainer's initial network for seed_no = seed: its draws from PyTorch's
+ * CPU generator after torch.manual_seed(seed) (DSP-WENO_network_trainer.py:
+ * 296) -- nn.Linear's default initialisation of each layer in order (biases
+ * U(+-1/sqrt(fan_in)); :54-56), then nn.init.xavier_uniform_ of every weight
+ * (:58-60) -- reproduced bit for bit (MT19937 on the low 32 bits of seed,
+ * PyTorch's 53-bit float64 uniform with a fused multiply-add). */
 int tecno_init_net(tecno_ctx* ctx, int n_layers, const int32_t* widths,
                    uint64_t seed);
 

@@ -138,6 +138,22 @@ def linadv_harvest(n, t):
     return harvest_stencils_1d(u, np.sin(xf - tt) ** 4)
 
 
+def test_init_net_matches_pytorch_draws():
+    """tecno_init_net through the C ABI: the weights and biases PyTorch draws
+    for the trainer's initialisation under torch.manual_seed(seed)
+    (DSP-WENO_network_trainer.py:54-60, 296), bit for bit, for the three
+    golden width / seed cases."""
+    fix = np.load(os.path.join(os.path.dirname(__file__), "golden", "fixtures_init.npz"))
+    s = net_solver()
+    for k in range(3):
+        widths = [int(x) for x in fix[f"widths{k}"]]
+        s.init_net(widths, int(fix[f"seed{k}"]))
+        w, W, b = s.get_net()
+        assert list(w) == widths
+        assert np.array_equal(W, fix[f"W{k}"]) and np.array_equal(b, fix[f"b{k}"])
+    s.close()
+
+
 def test_training_on_harvested_stencils_beats_spweno():
     """Xavier-initialised 5-5-5-5-5 network trained on the GPU (bf16
     activations, fp32 master weights, Adam 1e-3 / (0.5, 0.9) / 1e-5) on stencils

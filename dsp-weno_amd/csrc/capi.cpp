@@ -1338,6 +1338,18 @@ int tecno_save_net(const tecno_ctx* c, const char* path) {
   return ok ? 0 : fail(TECNO_EIO, std::string("write failed: ") + path);
 }
 
+int tecno_trainer_epoch_seeds(int nl, const int32_t* widths, uint64_t seed, int n_epochs,
+                              uint64_t* out) {
+  if (!widths || (!out && n_epochs > 0)) return fail(TECNO_EINVAL, "null argument");
+  if (nl < 1 || nl > TECNO_NET_MAXL) return fail(TECNO_EINVAL, "bad layer count");
+  if (n_epochs < 0) return fail(TECNO_EINVAL, "negative epoch count");
+  for (int l = 0; l <= nl; ++l)
+    if (widths[l] < 1 || widths[l] > TECNO_NET_MAXW)
+      return fail(TECNO_EINVAL, "layer widths must be 1..16");
+  trainer_epoch_seeds(nl, widths, seed, n_epochs, out);
+  return 0;
+}
+
 int tecno_init_net(tecno_ctx* c, int nl, const int32_t* widths, uint64_t seed) {
   if (!c || !widths) return fail(TECNO_EINVAL, "null argument");
   if (nl < 1 || nl > TECNO_NET_MAXL) return fail(TECNO_EINVAL, "bad layer count");

@@ -7,6 +7,7 @@
 #include <random>
 #include "fv_mixed.hpp"
 #include "mlp_mfma.hpp"
+#include "netinit.hpp"
 
 namespace tecno {
 
@@ -373,14 +374,14 @@ int MlpTrainer::epoch(int64_t batch, uint64_t seed, bool bf16, double* mean_loss
     err = "batch size must be positive";
     return -1;
   }
-  // seeded shuffle of the row order (the trainer's DataLoader(shuffle=True))
-  std::vector<int> order((size_t)ndata_);
-  for (int64_t i = 0; i < ndata_; ++i) order[(size_t)i] = (int)i;
-  std::mt19937_64 rng(seed);
-  for (int64_t i = ndata_ - 1; i > 0; --i) {
-    std::uniform_int_distribution<int64_t> d(0, i);
-    std::swap(order[(size_t)i], order[(size_t)d(rng)]);
+  if (ndata_ > TORCH_RANDPERM_MAX) {
+    err = "more training rows than the epoch shuffle takes (214748364)";
+    return -1;
   }
+  // the row order of the trainer's DataLoader(shuffle=True) whose sampler
+  // drew `seed`: torch.randperm under a generator seeded with it
+  std::vector<int> order;
+  randperm_torch(seed, ndata_, order);
   TCHK(hipMemcpyAsync(didx_, order.data(), (size_t)ndata_ * sizeof(int), hipMemcpyHostToDevice, s_));
   TCHK(hipMemsetAsync(dloss_, 0, 2 * sizeof(double), s_));
   for (int64_t r0 = 0; r0 < ndata_; r0 += batch) {

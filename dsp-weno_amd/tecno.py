@@ -195,6 +195,8 @@ def lib():
         L.tecno_train_epoch.argtypes = [P, I64, C.c_uint64, I32, dp]
         L.tecno_save_net.argtypes = [P, C.c_char_p]
         L.tecno_init_net.argtypes = [P, I32, C.POINTER(C.c_int32), C.c_uint64]
+        L.tecno_trainer_epoch_seeds.argtypes = [I32, C.POINTER(C.c_int32), C.c_uint64, I32,
+                                                C.POINTER(C.c_uint64)]
         L.tecno_ens_create.argtypes = [C.POINTER(Cfg), I32, C.POINTER(P)]
         L.tecno_ens_destroy.argtypes = [P]
         L.tecno_ens_destroy.restype = None
@@ -425,6 +427,16 @@ def read_param(path, level=0):
                                   C.byref(ts), C.byref(mi), buf, len(buf)))
     return cfg, dict(levels=nl.value, final_time=ft.value, time_stamps=ts.value, max_iter=mi.value,
                      net_file=buf.value.decode())
+
+
+def trainer_epoch_seeds(widths, seed, n_epochs):
+    """Shuffle seeds of the trainer's first n_epochs epochs for seed_no = seed
+    and these network widths (pass seeds[e] to Solver.train_epoch of epoch e
+    to take the trainer's row order)."""
+    w = (C.c_int32 * len(widths))(*[int(x) for x in widths])
+    out = (C.c_uint64 * max(int(n_epochs), 1))()
+    _check(lib().tecno_trainer_epoch_seeds(len(widths) - 1, w, int(seed), int(n_epochs), out))
+    return [int(out[i]) for i in range(int(n_epochs))]
 
 
 def make_train_rows(u4, target):

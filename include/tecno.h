@@ -325,12 +325,23 @@ int tecno_train_grad(tecno_ctx* ctx, const double* rows, int64_t n_rows,
 /* Keep n_rows training rows (n x 21, as above) on the device for
  * tecno_train_epoch. */
 int tecno_train_set_data(tecno_ctx* ctx, const double* rows, int64_t n_rows);
-/* One pass over the device data set in mini-batches of `batch` rows taken in
- * a seeded random order (DataLoader(shuffle=True), trainer:336-347; the last
- * batch may be shorter), one Adam step per batch.  loss_out = mean row loss
- * over the pass. */
+/* One pass over the device data set in mini-batches of `batch` rows (the
+ * last batch may be shorter), one Adam step per batch.  The rows are taken in
+ * the order of the trainer's DataLoader(shuffle=True) (trainer:348, 392)
+ * whose RandomSampler drew `seed`: torch.randperm(n_rows) under a generator
+ * seeded with it (at most 214748364 rows).  loss_out = mean row loss over the
+ * pass. */
 int tecno_train_epoch(tecno_ctx* ctx, int64_t batch, uint64_t seed,
                       int use_bf16, double* loss_out);
+/* The shuffle seeds of the trainer's first n_epochs epochs when it runs with
+ * seed_no = seed and a network of n_layers with these widths
+ * (DSP-WENO_network_trainer.py:296, 351, 384-392): after the network's
+ * initial draws (tecno_init_net), every epoch's DataLoader iterator draws its
+ * base seed and then its sampler's seed from PyTorch's global generator.
+ * Passing seeds_out[e] to tecno_train_epoch of epoch e takes the trainer's
+ * row order.  Host-only; no context or device needed. */
+int tecno_trainer_epoch_seeds(int n_layers, const int32_t* widths, uint64_t seed,
+                              int n_epochs, uint64_t* seeds_out);
 /* Build training rows from raw 4-cell stencils and targets on the device:
  * u4 (n x 4), target (n x 2) -> rows (n x 21), computing features and
  * vertices as Neural_Network::forward does (scaled stencil features,

@@ -8,7 +8,8 @@ import pytest
 
 import oracle_ctypes as orc
 from cases import fixtures
-from tecno import FP32, MLP_FP32, Solver, harvest_stencils_1d, make_cfg, make_train_rows
+from tecno import (FP32, MLP_FP32, Solver, harvest_stencils_1d, make_cfg, make_train_rows,
+                   trainer_epoch_seeds)
 
 pytestmark = pytest.mark.gpu
 NET_FILE = os.path.join(os.path.dirname(__file__), "golden", "dsp_weno_net.txt")
@@ -152,6 +153,43 @@ def test_init_net_matches_pytorch_draws():
         assert list(w) == widths
         assert np.array_equal(W, fix[f"W{k}"]) and np.array_equal(b, fix[f"b{k}"])
     s.close()
+
+
+def test_epoch_takes_the_dataloader_order():
+    """tecno_train_epoch with the trainer's epoch seeds (tecno_trainer_epoch_seeds)
+    equals Adam steps on the batches of the DataLoader's row order recorded from
+    PyTorch (golden epoch_orders: 1000 rows, batches of 64, the last one of 40;
+    trainer:348, 384-392)."""
+    fix = np.load(os.path.join(os.path.dirname(__file__), "golden", "fixtures_init.npz"))
+    orders = fix["epoch_orders"]
+    rng = np.random.default_rng(3)
+    u4, tg = smooth_stencils(rng, 1200)
+    rows = make_train_rows(u4[:1000], tg[:1000])
+    seeds = trainer_epoch_seeds([5, 5, 5, 5, 5], 6008, 2)
+    a, b = net_solver(), net_solver()
+    for s in (a, b):
+        s.init_net([5, 5, 5, 5, 5], 6008)
+        s.train_init(lr=1e-3, beta1=0.5, beta2=0.9, eps=1e-8, weight_decay=1e-5)
+    a.train_set_data(rows)
+    for e in range(2):
+        a.train_epoch(64, seed=seeds[e], use_bf16=True)
+        for r0 in range(0, 1000, 64):
+            b.train_step(rows[orders[e, r0:r0 + 64]], use_bf16=True)
+    _, Wa, ba = a.get_net()
+    _, Wb, bb = b.get_net()
+    np.testing.assert_allclose(Wa, Wb, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(ba, bb, rtol=1e-12, atol=0)
+    # a different epoch order gives different weights at this tolerance
+    c = net_solver()
+    c.init_net([5, 5, 5, 5, 5], 6008)
+    c.train_init(lr=1e-3, beta1=0.5, beta2=0.9, eps=1e-8, weight_decay=1e-5)
+    c.train_set_data(rows)
+    for e in range(2):
+        c.train_epoch(64, seed=seeds[e] + 1, use_bf16=True)
+    _, Wc, _ = c.get_net()
+    assert not np.allclose(Wa, Wc, rtol=1e-9, atol=0)
+    for s in (a, b, c):
+        s.close()
 
 
 def test_training_on_harvested_stencils_beats_spweno():

@@ -4,7 +4,11 @@ biases that torch.manual_seed(seed), one nn.Linear per layer and
 nn.init.xavier_uniform_ of every weight produce -- the trainer's
 initialisation sequence (DSP-WENO_network_trainer.py:54-60, 296) -- for the
 shipped 5-5-5-5-5 widths at seed 6008 and two other widths / seeds (one
-above 2^32).  Host build with g++, no GPU; bitwise."""
+above 2^32).  The same fixtures hold torch.randperm permutations under
+seeded generators and the row orders of the first three epochs of a
+DataLoader(shuffle=True) built around that network, as the trainer builds it
+(:348, 351, 384-392): randperm_torch and tecno_trainer_epoch_seeds (through
+the C ABI, host-only) reproduce them.  Host build with g++, no GPU; bitwise."""
 import os
 import subprocess
 import tempfile
@@ -22,6 +26,18 @@ HARNESS = r"""
 #include <cstdlib>
 #include "netinit.hpp"
 int main(int argc, char** argv) {
+  if (argv[1][0] == 'p') {  // p seed n: head 64, tail 64, sum i * p[i] mod 2^64
+    const uint64_t seed = std::strtoull(argv[2], nullptr, 10);
+    const int64_t n = std::strtoll(argv[3], nullptr, 10);
+    std::vector<int> p;
+    tecno::randperm_torch(seed, n, p);
+    uint64_t sum = 0;
+    for (int64_t i = 0; i < n; ++i) sum += (uint64_t)i * (uint64_t)p[(size_t)i];
+    for (int64_t i = 0; i < n; ++i)
+      if (i < 64 || i >= n - 64 || n <= 5000) std::printf("%d\n", p[(size_t)i]);
+    std::printf("%llu\n", (unsigned long long)sum);
+    return 0;
+  }
   const uint64_t seed = std::strtoull(argv[1], nullptr, 10);
   std::vector<int32_t> w;
   for (int i = 2; i < argc; ++i) w.push_back(std::atoi(argv[i]));
@@ -63,3 +79,32 @@ def test_init_net_seed_changes_draws(exe):
     b = subprocess.run([exe, "6009", "5", "5", "5"], check=True, capture_output=True,
                        text=True).stdout.split()
     assert a != b
+
+
+def _perm(exe, seed, n):
+    out = subprocess.run([exe, "p", str(seed), str(n)], check=True, capture_output=True,
+                         text=True).stdout.split()
+    return np.array([int(x) for x in out[:-1]]), int(out[-1])
+
+
+@pytest.mark.parametrize("k", range(sum(1 for k in FIX.files if k.startswith("perm_n"))))
+def test_randperm_matches_pytorch(exe, k):
+    n, seed = int(FIX[f"perm_n{k}"]), int(FIX[f"perm_seed{k}"])
+    p, total = _perm(exe, seed, n)
+    if f"perm{k}" in FIX.files:
+        assert np.array_equal(p, FIX[f"perm{k}"])
+    else:
+        assert np.array_equal(p[:64], FIX[f"perm_head{k}"])
+        assert np.array_equal(p[64:], FIX[f"perm_tail{k}"])
+        assert total == int(FIX[f"perm_sum{k}"])
+
+
+def test_trainer_epoch_orders_match_dataloader(exe):
+    """tecno_trainer_epoch_seeds (C ABI, host-only) + randperm_torch give the
+    row order of each of the trainer's DataLoader epochs."""
+    import tecno
+    orders = FIX["epoch_orders"]
+    seeds = tecno.trainer_epoch_seeds([5, 5, 5, 5, 5], 6008, len(orders))
+    for e, sd in enumerate(seeds):
+        p, _ = _perm(exe, sd, orders.shape[1])
+        assert np.array_equal(p, orders[e]), e

@@ -750,9 +750,14 @@ struct Euler2Mixed {
       // chains interleave (127 VGPRs at the four waves per SIMD that LDS
       // allows; KH 2048^2 stage 0.38 -> 0.37 ms), where all four at once
       // spill (0.46 ms)
-#pragma unroll 2
-      for (int i = 0; i < 4; ++i) {
-        const float d1z = sel4(i, dZ[0][1], dZ[1][1], dZ[2][1], dZ[3][1]);
+#pragma unroll 1
+      for (int p = 0; p < 2; ++p) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        // component i = 2p + h: one select per value between the pair's
+        // components (p is the same on every lane)
+        const int i = 2 * p + h;
+        const float d1z = p ? dZ[2 + h][1] : dZ[h][1];
         const bool skip = !(fabsf(d1z) >= 1e-15f);
         // a component whose centre jump is zero on every lane of the wave
         // needs no network: its reconstructed jump is 0 (reconstruct.cc:152)
@@ -765,9 +770,9 @@ struct Euler2Mixed {
         // features are finite for any finite state: tanh of an overflowing
         // ratio saturates to +-1.)
         const float d1 = skip ? 1.0f : d1z;
-        const float d0 = sel4(i, dZ[0][0], dZ[1][0], dZ[2][0], dZ[3][0]);
-        const float d2 = sel4(i, dZ[0][2], dZ[1][2], dZ[2][2], dZ[3][2]);
-        const float z1 = sel4(i, Z1[0], Z1[1], Z1[2], Z1[3]);
+        const float d0 = p ? dZ[2 + h][0] : dZ[h][0];
+        const float d2 = p ? dZ[2 + h][2] : dZ[h][2];
+        const float z1 = p ? Z1[2 + h] : Z1[h];
         const float z2 = z1 + d1, z0 = z1 - d0, z3 = z2 + d2;
         const float maxabs = fmaxf(fmaxf(fabsf(z0), fabsf(z1)), fmaxf(fabsf(z2), fabsf(z3)));
         float st[5], lg[5], tm, tp, maxd;
@@ -776,8 +781,9 @@ struct Euler2Mixed {
         float j = fast::sign_round(fast::head_combine(lg, tm, tp, maxd, d0, d1, d2), d0, d1, d2);
         if (skip) j = 0.0f;
         bad |= (j * d1 < 0.0f) ? (1u << i) : 0u;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) jmp[k] = (k == i) ? j : jmp[k];
+        jmp[h] = p ? jmp[h] : j;
+        jmp[2 + h] = p ? j : jmp[2 + h];
+      }
       }
     } else
 #endif

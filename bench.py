@@ -530,8 +530,12 @@ def load_traffic(variant):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    # each path starts from a fresh solver (allocation, state upload); its
+    # first steps run while the GPU clocks come back up (stage 0.345 ms after
+    # 5 untimed steps, 0.326 ms after 30), so 30 untimed steps precede the
+    # timed ones
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="",

@@ -539,7 +539,8 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="",
-                    help="headline | fp64 | fp64dsp (profiling runs) | matrix (network variants)")
+                    help="headline | fp64 | fp64dsp | mixedspw (profiling runs) | matrix (network "
+                         "variants)")
     ap.add_argument("--ensemble", action="store_true",
                     help="run the 48-problem scalar ensemble (batched, hipGraph) and exit")
     ap.add_argument("--no-graph", action="store_true", help="ensemble without the hipGraph")
@@ -644,7 +645,8 @@ def main():
              # SP-WENO with the headline's arithmetic split, no network
              "mixed_spweno": (tecno.FP32, "sp_weno", tecno.MLP_FP64, "shipped")}
     sel = {"": tuple(paths), "headline": ("headline",), "fp64": ("fp64_spweno",),
-           "fp64dsp": ("fp64_dsp",), "matrix": tuple(paths)}[args.only]
+           "fp64dsp": ("fp64_dsp",), "mixedspw": ("mixed_spweno",),
+           "matrix": tuple(paths)}[args.only]
     for name in sel:
         prec, rec, mlp, weights = paths[name]
         s = make_solver(tecno, n, prec, rec, mlp, weights)

@@ -70,50 +70,44 @@ __device__ __forceinline__ float logavg(float a, float b) {
 }
 
 // ---------------------------------------------------------------- SP-WENO
-__device__ __forceinline__ float spw_cp(float t0, float t1, float del1, float scale, bool wc,
-                                        float& corr) {
-  const float eps = 1.0e-13f;
-  float psi;
-  if (fabsf(t0 - 1.0f) < eps || fabsf(t1 - 1.0f) < eps)
-    psi = 0.0f;
-  else
-    psi = quot(1.0f - t1, 1.0f - t0);
-  float C = 0.0f;
-  corr = 0.0f;
-  if (fabsf(psi + 1.0f) > eps && fabsf(t0 - 1.0f) > eps && psi < 0.0f) {
-    // Cp's a / (a^2 + b^2) / 8 (limiter.h:152-190) with a = fp(t0, t1) =
-    // 1 / (1 + psi) and b = fp(t1, t0) = 1 / (1 + 1/psi) (limiter.h:103) is
-    // (1 + psi) / (1 + psi^2) / 8: one reciprocal instead of two quotients
-    // and three reciprocals
-    C = 0.125f * (1.0f + psi) * rcp(fmaf(psi, psi, 1.0f));
-    if (wc) {
-      const float m = fminf(fabsf(del1 * rcp(scale)), fabsf(del1));
-      corr = -0.25f * m * m * m * rcp(1.0f - t0);
-    }
-  } else if (fabsf(t0 - 1.0f) > eps && fabsf(psi + 1.0f) < eps) {
-    C = 0.0f;
-  } else if (fabsf(t0 - 1.0f) < eps || (psi > -eps && fabsf(t0) <= 1.0f)) {
-    // |t0| < 1 + eps of the FP64 test: 1 + 1e-13 rounds to 1 in FP32, and
-    // the floats below 1 + 1e-13 are those <= 1 (|t0| = 1 exactly: equal or
-    // alternating jumps)
-    C = -0.375f;
-  } else if (psi > -eps && fabsf(t0) > 1.0f) {
-    C = 0.125f;
-  }
-  return C;
-}
-
-// reconstructed jump of SP-WENO from the cell jumps (u1, u2 only for sp_wenoc)
+// Reconstructed jump of SP-WENO from the cell jumps (u1, u2 only for
+// sp_wenoc): sp_weno_rec (limiter.h:193-252) with C1 = Cp(t+, t-) and
+// C2 = Cp(t-, t+) (limiter.h:152-190) from one ratio psi = (1 - t-)/(1 - t+):
+// Cp's psi for C2 is 1/psi, so both take Cp's first branch together
+// (psi < 0), where Cp's a / (a^2 + b^2) / 8 with a = fp(t0, t1) =
+// 1 / (1 + psi) and b = fp(t1, t0) = 1 / (1 + 1/psi) (limiter.h:103) is
+// (1 + psi) / (1 + psi^2) / 8 for C1 and psi times that for C2.  Otherwise
+// (psi >= 0, or a ratio equal to 1) Cp is -3/8 for |t0| < 1 + eps and 1/8
+// above; in FP32 1 + 1e-13 rounds to 1, so the floats below 1 + 1e-13 are
+// those <= 1 (|t0| = 1 exactly: equal or alternating jumps).
 __device__ __forceinline__ float spw_jump(float d0, float d1, float d2, float u1, float u2,
                                           int corr) {
-  if (fabsf(d1) < 1.0e-13f) return d1;  // limiter.h:202-206: (u1, u2)
+  const float eps = 1.0e-13f;
+  if (fabsf(d1) < eps) return d1;  // limiter.h:202-206: (u1, u2)
   const float tp = quot(d0, d1), tm = quot(d2, d1);
-  const bool wc = corr == 1;
-  const float scale = 0.5f * (fabsf(u1) + fabsf(u2));
-  float c1c, c2c;
-  float C1 = spw_cp(tp, tm, d1, scale, wc, c1c);
-  float C2 = spw_cp(tm, tp, d1, scale, wc, c2c);
-  if (wc) {
+  const bool one = fabsf(tp - 1.0f) < eps || fabsf(tm - 1.0f) < eps;
+  const float psi = one ? 0.0f : quot(1.0f - tm, 1.0f - tp);
+  float C1, C2, c1c = 0.0f, c2c = 0.0f;
+  if (psi < 0.0f) {
+    if (fabsf(psi + 1.0f) < eps) {  // Cp's second branch (psi = -1)
+      C1 = 0.0f;
+      C2 = 0.0f;
+    } else {
+      C1 = 0.125f * (1.0f + psi) * rcp(fmaf(psi, psi, 1.0f));
+      C2 = psi * C1;
+      if (corr == 1) {
+        const float scale = 0.5f * (fabsf(u1) + fabsf(u2));
+        const float m = fminf(fabsf(d1 * rcp(scale)), fabsf(d1));
+        const float m3 = -0.25f * m * m * m;
+        c1c = m3 * rcp(1.0f - tp);
+        c2c = m3 * rcp(1.0f - tm);
+      }
+    }
+  } else {
+    C1 = fabsf(tp) <= 1.0f ? -0.375f : 0.125f;
+    C2 = fabsf(tm) <= 1.0f ? -0.375f : 0.125f;
+  }
+  if (corr == 1) {
     C1 = fminf(fmaxf(C1 + c1c, -0.375f), 0.125f);
     C2 = fminf(fmaxf(C2 + c2c, -0.375f), 0.125f);
   }

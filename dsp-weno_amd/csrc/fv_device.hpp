@@ -180,7 +180,42 @@ __device__ __forceinline__ void spw_cp(T t0, T t1, T del1, T scale, bool want_co
   }
 }
 
-// sp_weno_rec (limiter.h:193-252); corr 0 sp_weno, 1 sp_wenoc, 2 corr2 (== 0)
+// FP32 form of the two Cp calls of sp_weno_rec from one ratio
+// psi = (1 - t-)/(1 - t+): C2's psi is 1/psi, so both weights take Cp's first
+// branch together (psi < 0), where a / (a^2 + b^2) / 8 with a = fp(t0, t1),
+// b = fp(t1, t0) is (1 + psi) / (1 + psi^2) / 8 for C1 and psi times that
+// for C2 (limiter.h:103, 152-190); otherwise Cp is -3/8 for |t0| <= 1 (the
+// floats below 1 + 1e-13) and 1/8 above.  Four divisions instead of ten.
+template <typename T>
+__device__ __forceinline__ void spw_c12_ratio(T tp, T tm, T del1, T scale, bool want_corr, T& C1,
+                                              T& C2, T& C1c, T& C2c) {
+  const T eps = T(SpwConst<T>::eps);
+  const bool one = m_abs(tp - T(1)) < eps || m_abs(tm - T(1)) < eps;
+  const T psi = one ? T(0) : (T(1) - tm) / (T(1) - tp);
+  C1c = T(0);
+  C2c = T(0);
+  if (psi < T(0)) {
+    if (m_abs(psi + T(1)) < eps) {  // Cp's second branch (psi = -1)
+      C1 = T(0);
+      C2 = T(0);
+    } else {
+      C1 = T(0.125) * (T(1) + psi) / (psi * psi + T(1));
+      C2 = psi * C1;
+      if (want_corr) {
+        const T m3 = T(-0.25) * m_cube(m_min(m_abs(del1 / scale), m_abs(del1)));
+        C1c = m3 / (T(1) - tp);
+        C2c = m3 / (T(1) - tm);
+      }
+    }
+  } else {
+    C1 = m_abs(tp) <= T(1) ? T(-3.0 / 8.0) : T(1.0 / 8.0);
+    C2 = m_abs(tm) <= T(1) ? T(-3.0 / 8.0) : T(1.0 / 8.0);
+  }
+}
+
+// sp_weno_rec (limiter.h:193-252); corr 0 sp_weno, 1 sp_wenoc, 2 corr2 (== 0).
+// FP64 keeps the reference's two Cp calls (its operation order); FP32 takes
+// spw_c12_ratio.
 template <typename T>
 __device__ __forceinline__ void sp_weno(T s0, T s1, T s2, T s3, int corr, T& recl, T& recr) {
   const T eps = T(SpwConst<T>::eps);
@@ -195,8 +230,12 @@ __device__ __forceinline__ void sp_weno(T s0, T s1, T s2, T s3, int corr, T& rec
   T scale = T(0.5) * (m_abs(s1) + m_abs(s2));
   T C1 = 0, C2 = 0, C1c = 0, C2c = 0;
   bool wc = (corr == 1);
-  spw_cp(tp, tm, del1, scale, wc, C1, C1c);
-  spw_cp(tm, tp, del1, scale, wc, C2, C2c);
+  if constexpr (sizeof(T) == 4) {
+    spw_c12_ratio(tp, tm, del1, scale, wc, C1, C2, C1c, C2c);
+  } else {
+    spw_cp(tp, tm, del1, scale, wc, C1, C1c);
+    spw_cp(tm, tp, del1, scale, wc, C2, C2c);
+  }
   if (corr > 0) {
     if (corr == 1) {
       C1 = C1 + C1c;

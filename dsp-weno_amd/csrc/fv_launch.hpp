@@ -523,6 +523,12 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
     if (L.fast_flux == FV_KEPES4_ROE && L.fast_rec == RV_DSP)
       return launch_stage_t<Euler2Ph<T, 2, FV_KEPES4_ROE, RV_DSP>, 2, XD, YD, 2>(L, P, U, U0, Uo,
                                                                                  k);
+    // the other TeCNO fluxes with the SP-WENO family: the generic face built
+    // with that reconstruction only (no ENO, network or MUSCL code, far fewer
+    // registers), same arithmetic as the general kernel's SP-WENO branch
+    if (L.fast_rec == RV_SPWENO)
+      return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_SPWENO>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
+                                                                                k);
     return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
                                                                                k);
   }

@@ -513,6 +513,9 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
       default:
         break;
     }
+    // ENO k = 3 takes 3 ghost layers; ENO alone in the face as below
+    if (L.G == 3 && P.recon == REC_ENO)
+      return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_ENO>, 2, X2, Y2, 3>(L, P, U, U0, Uo, k);
     if (L.G == 3)
       return launch_stage_t<Euler2Ph<T, 3, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 3>(L, P, U, U0, Uo,
                                                                                  k);
@@ -529,6 +532,10 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
     if (L.fast_rec == RV_SPWENO)
       return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_SPWENO>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
                                                                                 k);
+    if (L.fast_rec == RV_DSP)
+      return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_DSP>, 2, X2, Y2, 2>(L, P, U, U0, Uo, k);
+    if (P.recon == REC_ENO)
+      return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_ENO>, 2, X2, Y2, 2>(L, P, U, U0, Uo, k);
     return launch_stage_t<Euler2Ph<T, 2, FV_GENERIC, RV_GENERIC>, 2, X2, Y2, 2>(L, P, U, U0, Uo,
                                                                                k);
   }

@@ -262,7 +262,7 @@ __device__ __forceinline__ void roe2(const Phys<T>& P, T rl, T nl, T tl, T pl, T
 }
 
 enum { FV_KEPES4_ROE = 0, FV_GENERIC = 1 };
-enum { RV_SPWENO = 0, RV_DSP = 1, RV_GENERIC = 2 };
+enum { RV_SPWENO = 0, RV_DSP = 1, RV_GENERIC = 2, RV_ENO = 3 };
 
 // FP32 jumps of NC characteristic components (defined in fv_mixed.hpp)
 template <int MIX, int NC>
@@ -520,6 +520,11 @@ struct Euler2Ph {
         dsp_rec(P.net, a5, Z1[L - 1], Z1[L], Z1[R], Z1[R + 1], l_, r_); dZ1 = r_ - l_;
         dsp_rec(P.net, a5, Z2[L - 1], Z2[L], Z2[R], Z2[R + 1], l_, r_); dZ2 = r_ - l_;
         dsp_rec(P.net, a5, Z4[L - 1], Z4[L], Z4[R], Z4[R + 1], l_, r_); dZ4 = r_ - l_;
+      } else if (RV == RV_ENO) {
+        eno_rec(P.eno_k, Z0, G, l_, r_); dZ0 = r_ - l_;
+        eno_rec(P.eno_k, Z1, G, l_, r_); dZ1 = r_ - l_;
+        eno_rec(P.eno_k, Z2, G, l_, r_); dZ2 = r_ - l_;
+        eno_rec(P.eno_k, Z4, G, l_, r_); dZ4 = r_ - l_;
       } else {
         recon_any(P, Z0, G, l_, r_); dZ0 = r_ - l_;
         recon_any(P, Z1, G, l_, r_); dZ1 = r_ - l_;

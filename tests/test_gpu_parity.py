@@ -365,6 +365,36 @@ def test_euler2d_flux_variants_vs_oracle(flux):
     assert rel_err(s.get_state(), o.get_state()[[0, 1, 2, 4]]) <= 3e-12
 
 
+@pytest.mark.parametrize("flux,rec", [("kepes_tecno_roe", "eno 2"), ("kepes_tecno_roe", "eno 3"),
+                                      ("roe_tecno4_roe", "eno 3"), ("kepes_tecno4_rusanov", "eno 2"),
+                                      ("kepes_tecno_roe", "sp_weno_dl"),
+                                      ("roe_tecno4_roe", "sp_weno_dl"),
+                                      ("kepes_tecno_roe", "sp_wenoc"), ("kepes_tecno_roe", "second"),
+                                      ("kepes_tecno_roe", "first")])
+def test_euler2d_generic_face_reconstructions_vs_oracle(flux, rec):
+    """FP64 2D Euler, the TeCNO fluxes other than kepes_tecno4_roe with ENO k = 2, 3,
+    DSP-WENO, sp_wenoc and the general face's remaining schemes, three steps
+    against the oracle (the ENO, DSP-WENO and SP-WENO family runs take the
+    generic face built with that reconstruction alone)."""
+    rng = np.random.default_rng(17)
+    nx, ny = 36, 29
+    cfg = make_cfg(n=(nx, ny, 2), dims=2, recon=rec, flux=flux, bc="periodic")
+    dsp = rec == "sp_weno_dl"
+    # smooth data for the network (its FP64 tanh / exp against the C library's)
+    prim = _random_euler_state(rng, nx, ny, dsp)
+    o = orc.OracleSolver(cfg, orc.load_net() if dsp else None, faithful=False)
+    full = np.zeros((5, nx * ny))
+    full[[0, 1, 2, 4]] = prim
+    o.set_state(full)
+    s = solver_for(cfg, dsp)
+    s.set_state(prim)
+    for step in range(3):
+        o.step()
+        s.step()
+    assert rel_err(s.get_state(), o.get_state()[[0, 1, 2, 4]]) <= 3e-12
+    s.close()
+
+
 @pytest.mark.parametrize("flux", ["lin_tecno4", "bur_tecno4", "lin_tecno", "bur_ec4"])
 @pytest.mark.parametrize("rec", ["sp_weno", "eno 3", "sp_weno_dl", "second", "minmod", "minabs"])
 def test_scalar2d_per_step_vs_oracle(flux, rec):

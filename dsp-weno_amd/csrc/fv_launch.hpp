@@ -437,15 +437,19 @@ hipError_t Launch<T>::stage(const LaunchInfo& L, const Phys<T>& P, const void* U
         return hipErrorNotSupported;
       }
     }
-    // the SP-WENO family runs the kernel with that reconstruction only
-    const bool spw = P.recon == REC_SPWENO;
+    // the SP-WENO family and ENO run the kernel with that reconstruction only
+    const bool spw = P.recon == REC_SPWENO, eno = P.recon == REC_ENO;
     if (L.dims == 1)
-      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 1, X1, 1, 3>(L, P, U, U0, Uo, k)
+      return L.G == 3 ? (eno ? launch_stage_t<ScalarPh<T, 3, 2>, 1, X1, 1, 3>(L, P, U, U0, Uo, k)
+                             : launch_stage_t<ScalarPh<T, 3>, 1, X1, 1, 3>(L, P, U, U0, Uo, k))
              : spw    ? launch_stage_t<ScalarPh<T, 2, 1>, 1, X1, 1, 2>(L, P, U, U0, Uo, k)
+             : eno    ? launch_stage_t<ScalarPh<T, 2, 2>, 1, X1, 1, 2>(L, P, U, U0, Uo, k)
                       : launch_stage_t<ScalarPh<T, 2>, 1, X1, 1, 2>(L, P, U, U0, Uo, k);
     if (L.dims == 2)
-      return L.G == 3 ? launch_stage_t<ScalarPh<T, 3>, 2, X2, Y2, 3>(L, P, U, U0, Uo, k)
+      return L.G == 3 ? (eno ? launch_stage_t<ScalarPh<T, 3, 2>, 2, X2, Y2, 3>(L, P, U, U0, Uo, k)
+                             : launch_stage_t<ScalarPh<T, 3>, 2, X2, Y2, 3>(L, P, U, U0, Uo, k))
              : spw    ? launch_stage_t<ScalarPh<T, 2, 1>, 2, X2, Y2, 2>(L, P, U, U0, Uo, k)
+             : eno    ? launch_stage_t<ScalarPh<T, 2, 2>, 2, X2, Y2, 2>(L, P, U, U0, Uo, k)
                       : launch_stage_t<ScalarPh<T, 2>, 2, X2, Y2, 2>(L, P, U, U0, Uo, k);
     return hipErrorNotSupported;
   }

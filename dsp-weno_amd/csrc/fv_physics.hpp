@@ -12,7 +12,8 @@ namespace tecno {
 // sp_wenoc, sp_weno_corr2: P.corr), for problems that use it.  Without the
 // other reconstructions compiled in -- the FP64 network above all -- the FP64
 // 2D kernels need 88-92 VGPRs instead of 161-166 and run five waves per SIMD
-// instead of three, which their chains of FP64 divisions need.
+// instead of three, which their chains of FP64 divisions need.  RF 2: ENO
+// (k = 2, 3) only, in the same way.
 template <typename T, int G, int RF = 0>
 struct ScalarPh {
   using TS = T;  // state in HBM
@@ -81,6 +82,8 @@ struct ScalarPh {
       T uij, uji;
       if constexpr (RF == 1)
         sp_weno(s[G - 2], s[G - 1], s[G], s[G + 1], P.corr, uij, uji);
+      else if constexpr (RF == 2)
+        eno_rec(P.eno_k, s, G, uij, uji);
       else
         recon_any(P, s, G, uij, uji);
       f -= T(0.5) * D * (uji - uij);
